@@ -1,0 +1,232 @@
+"""Python binding of the gfx950 MHD2D library (C ABI: include/mhd2d.h).
+
+The class :class:`Solver` keeps the names and meaning of the reference's
+Fortran subroutines (GDF_Balsara_P2_0316: ``set_bc``, ``Lh``, ``LhEz``,
+``TVB_Limiter_P2``, ``pp_Limiter``, ``KXRCF_Detector``, ``div_free_Balsara``,
+``calculate_dt``, the diagnostics and ``RK3``) over one device context.
+Host arrays use the reference layouts (column-major):
+``uh (Nx+2, Ny+2, 10, 8)``, ``Bx (Nx+1, Ny, 3)``, ``By (Nx, Ny+1, 3)``.
+
+There is no CPU fallback: if ``libmhd2d.so`` is missing or no GPU is
+visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmhd2d.so")
+
+LF, HLL, HLLC, HLLD = 1, 2, 3, 4
+IND_NONE, IND_ALL, IND_KXRCF, IND_MLP = 0, 1, 2, 3
+FP64, MIXED = 0, 1
+IC_VORTEX, IC_OT, IC_ROTOR, IC_OT_PERTURBED, IC_RIEMANN4 = 1, 3, 4, 103, 104
+DU, DEZ1, DEZ2, EZVERTEX, TROUBLE = 0, 1, 2, 3, 4
+
+# exported symbols of include/mhd2d.h
+EXPORTS = (
+    "mhd2d_create", "mhd2d_destroy", "mhd2d_last_error", "mhd2d_sync", "mhd2d_upload",
+    "mhd2d_download", "mhd2d_init_data", "mhd2d_set_bc", "mhd2d_lh", "mhd2d_lhez",
+    "mhd2d_rk_stage", "mhd2d_tvb_limiter_p2", "mhd2d_pp_limiter", "mhd2d_kxrcf_detector",
+    "mhd2d_div_free_balsara", "mhd2d_calculate_dt", "mhd2d_diagnostics", "mhd2d_rk3_prologue",
+    "mhd2d_step", "mhd2d_get_time", "mhd2d_set_time", "mhd2d_debug_get",
+)
+
+
+class Config(C.Structure):
+    """mhd2d_config (include/mhd2d.h)."""
+    _fields_ = [
+        ("Nx", C.c_int32), ("Ny", C.c_int32), ("flux_type", C.c_int32),
+        ("bcR", C.c_int32), ("bcL", C.c_int32), ("bcU", C.c_int32), ("bcD", C.c_int32),
+        ("pp", C.c_int32), ("indicator", C.c_int32), ("precision", C.c_int32), ("reserved", C.c_int32),
+        ("xa", C.c_double), ("xb", C.c_double), ("ya", C.c_double), ("yb", C.c_double),
+        ("M", C.c_double), ("CFL", C.c_double), ("Ck", C.c_double),
+        ("mlp_lo", C.c_double), ("mlp_hi", C.c_double),
+    ]
+
+
+_lib = None
+
+
+class MHD2DError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """dlopen libmhd2d.so and declare the C ABI; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MHD2DError(f"{path} is not built (run __graft_entry__.build() or make -C {HERE})")
+    L = C.CDLL(path)
+    P, dp, i32, i64p = C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int64)
+    sig = {
+        "mhd2d_create": ([C.POINTER(Config), C.POINTER(C.c_void_p)], C.c_int),
+        "mhd2d_destroy": ([P], C.c_int),
+        "mhd2d_last_error": ([P], C.c_char_p),
+        "mhd2d_sync": ([P], C.c_int),
+        "mhd2d_upload": ([P, dp, dp, dp], C.c_int),
+        "mhd2d_download": ([P, dp, dp, dp], C.c_int),
+        "mhd2d_init_data": ([P, i32, dp, i32], C.c_int),
+        "mhd2d_set_bc": ([P], C.c_int),
+        "mhd2d_lh": ([P], C.c_int),
+        "mhd2d_lhez": ([P], C.c_int),
+        "mhd2d_rk_stage": ([P, i32, C.c_double], C.c_int),
+        "mhd2d_tvb_limiter_p2": ([P], C.c_int),
+        "mhd2d_pp_limiter": ([P], C.c_int),
+        "mhd2d_kxrcf_detector": ([P, i64p], C.c_int),
+        "mhd2d_div_free_balsara": ([P], C.c_int),
+        "mhd2d_calculate_dt": ([P, dp], C.c_int),
+        "mhd2d_diagnostics": ([P, dp, dp, dp], C.c_int),
+        "mhd2d_rk3_prologue": ([P], C.c_int),
+        "mhd2d_step": ([P, C.c_double, dp], C.c_int),
+        "mhd2d_get_time": ([P, dp, dp, i64p], C.c_int),
+        "mhd2d_set_time": ([P, C.c_double, C.c_int64], C.c_int),
+        "mhd2d_debug_get": ([P, i32, dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Solver:
+    """One device context: the reference's COMMON state on the GPU."""
+
+    def __init__(self, Nx, Ny=None, flux_type=HLLD, bc=1, domain=(0.0, 2 * np.pi, 0.0, 2 * np.pi),
+                 M=10.0, CFL=0.15, pp=0, indicator=IND_ALL, precision=FP64, Ck=1.0,
+                 mlp_band=(0.2, 0.8)):
+        self.L = load_library()
+        Ny = Nx if Ny is None else Ny
+        self.Nx, self.Ny = int(Nx), int(Ny)
+        bcs = (bc, bc, bc, bc) if np.isscalar(bc) else tuple(bc)
+        self.cfg = Config(self.Nx, self.Ny, flux_type, *bcs, pp, indicator, precision, 0,
+                          *domain, M, CFL, Ck, *mlp_band)
+        h = C.c_void_p()
+        rc = self.L.mhd2d_create(C.byref(self.cfg), C.byref(h))
+        if rc != 0:
+            raise MHD2DError(f"mhd2d_create failed with status {rc}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mhd2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.mhd2d_last_error(self.h).decode(errors="replace")
+            raise MHD2DError(f"{what}: status {rc}: {msg}")
+        return rc
+
+    # --- state transfer -------------------------------------------------
+    def shapes(self):
+        return ((self.Nx + 2, self.Ny + 2, 10, 8), (self.Nx + 1, self.Ny, 3), (self.Nx, self.Ny + 1, 3))
+
+    def upload(self, uh, Bx, By):
+        uh = np.asfortranarray(uh, dtype=np.float64)
+        Bx = np.asfortranarray(Bx, dtype=np.float64)
+        By = np.asfortranarray(By, dtype=np.float64)
+        su, sx, sy = self.shapes()
+        if uh.shape != su or Bx.shape != sx or By.shape != sy:
+            raise ValueError(f"shapes {uh.shape} {Bx.shape} {By.shape} != {su} {sx} {sy}")
+        self._chk(self.L.mhd2d_upload(self.h, _dp(uh), _dp(Bx), _dp(By)), "upload")
+
+    def download(self):
+        su, sx, sy = self.shapes()
+        uh = np.zeros(su, order="F")
+        Bx = np.zeros(sx, order="F")
+        By = np.zeros(sy, order="F")
+        self._chk(self.L.mhd2d_download(self.h, _dp(uh), _dp(Bx), _dp(By)), "download")
+        return uh, Bx, By
+
+    def debug_get(self, which):
+        su, sx, sy = self.shapes()
+        shape = {DU: su, DEZ1: sx, DEZ2: sy, EZVERTEX: (self.Nx + 1, self.Ny + 1), TROUBLE: (self.Nx, self.Ny)}[which]
+        a = np.zeros(shape, order="F")
+        self._chk(self.L.mhd2d_debug_get(self.h, which, _dp(a)), "debug_get")
+        return a
+
+    # --- the reference's subroutines ------------------------------------
+    def init_data(self, ic, extra=()):
+        ex = np.ascontiguousarray(extra, dtype=np.float64)
+        buf = ex if ex.size else np.zeros(1)
+        self._chk(self.L.mhd2d_init_data(self.h, ic, _dp(buf), int(ex.size)), "init_data")
+
+    def set_bc(self):
+        self._chk(self.L.mhd2d_set_bc(self.h), "set_bc")
+
+    def Lh(self):
+        self._chk(self.L.mhd2d_lh(self.h), "Lh")
+
+    def LhEz(self):
+        self._chk(self.L.mhd2d_lhez(self.h), "LhEz")
+
+    def rk_stage(self, stage, dt):
+        self._chk(self.L.mhd2d_rk_stage(self.h, stage, dt), "rk_stage")
+
+    def TVB_Limiter_P2(self):
+        self._chk(self.L.mhd2d_tvb_limiter_p2(self.h), "TVB_Limiter_P2")
+
+    def pp_Limiter(self):
+        self._chk(self.L.mhd2d_pp_limiter(self.h), "pp_Limiter")
+
+    def KXRCF_Detector(self):
+        n = C.c_int64()
+        self._chk(self.L.mhd2d_kxrcf_detector(self.h, C.byref(n)), "KXRCF_Detector")
+        return n.value
+
+    def div_free_Balsara(self):
+        self._chk(self.L.mhd2d_div_free_balsara(self.h), "div_free_Balsara")
+
+    def calculate_dt(self):
+        d = C.c_double()
+        self._chk(self.L.mhd2d_calculate_dt(self.h, C.byref(d)), "calculate_dt")
+        return d.value
+
+    def diagnostics(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self.L.mhd2d_diagnostics(self.h, C.byref(a), C.byref(b), C.byref(c)), "diagnostics")
+        return {"umax": a.value, "totaldiv": b.value, "pmin": c.value}
+
+    def rk3_prologue(self):
+        self._chk(self.L.mhd2d_rk3_prologue(self.h), "rk3_prologue")
+
+    def step(self, tend=float("inf"), sync=False):
+        t = C.c_double()
+        self._chk(self.L.mhd2d_step(self.h, tend, C.byref(t) if sync else None), "step")
+        return t.value if sync else None
+
+    def get_time(self):
+        t, dt, n = C.c_double(), C.c_double(), C.c_int64()
+        self._chk(self.L.mhd2d_get_time(self.h, C.byref(t), C.byref(dt), C.byref(n)), "get_time")
+        return t.value, dt.value, n.value
+
+    def set_time(self, t, count=0):
+        self._chk(self.L.mhd2d_set_time(self.h, t, count), "set_time")
+
+    def sync(self):
+        self._chk(self.L.mhd2d_sync(self.h), "sync")
+
+    def RK3(self, tend, max_steps=None):
+        """RK3.f90:5-174 without frame output: prologue, then steps while t < tend."""
+        self.rk3_prologue()
+        t = 0.0
+        n = 0
+        while t < tend and (max_steps is None or n < max_steps):
+            t = self.step(tend, sync=True)
+            n += 1
+        return t, n

@@ -1,0 +1,1558 @@
+// mhd2d.hip -- gfx950 kernels and the C ABI (include/mhd2d.h) for one time
+// step of the 2D ideal-MHD P2 globally divergence-free DG solver of
+// GDF_Balsara_P2_0316.
+//
+// Device state: the 52 active (mode, variable) planes of uh, each an
+// (Nx+2) x (Ny+2) array with halos (so set_bc and the stale-halo reads of
+// the TVB limiter keep the reference's exact meaning), plus 6 face planes
+// (Bx modes 1-3 on x faces, By modes 1-3 on y faces).  Cell (i, j) of any
+// plane sits at j*P + OFF + i (mhd2d_tables.hpp).
+//
+// Kernels per RK stage (fused path, mhd2d_step):
+//   k_xface / k_yface  one lane per (face, Gauss point): traces, physical
+//                      fluxes, Davis bounds, 1D Riemann flux, face EMF
+//   k_vertex           one lane per vertex: 2D Riemann (corner) EMF
+//   k_cell<MODE>       one lane per cell: volume + surface integrals, mass
+//                      inverse, SSP-RK3 combination (RK3.f90:157-212)
+//   k_ct<MODE>         one lane per face: CT right-hand side + combination
+//   k_limit            one lane per cell: TVB (+ positivity) + Balsara
+//                      divergence-free reconstruction
+//   k_setbc            halo ring
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/mhd2d.h"
+#include "mhd2d_physics.hpp"
+#include "mhd2d_tables.hpp"
+#include "mhd2d_tvb.hpp"
+
+using namespace mhd;
+
+namespace {
+
+constexpr int BS = 256;
+
+// device scalars of the time loop (COMMON /time/ and the diagnostics)
+struct DevScal {
+  double t, dt, tend, CFL;
+  double alphax, alphay, umax, pmin, totaldiv;
+  unsigned long long ax_bits, ay_bits, umax_bits;
+  long long pmin_key;
+  long long count;
+  long long nflag;
+  int active;
+  int pad;
+};
+
+__device__ __forceinline__ long long dkey(double x) {   // order-preserving key of a double
+  long long b = __double_as_longlong(x);
+  return b >= 0 ? b : b ^ 0x7fffffffffffffffLL;
+}
+__device__ __forceinline__ double dunkey(long long k) {
+  return __longlong_as_double(k >= 0 ? k : k ^ 0x7fffffffffffffffLL);
+}
+
+// blocks of consecutive ids go to the same XCD (round-robin dispatch over 8)
+__device__ __forceinline__ long long xcd_block() {
+  const unsigned b = blockIdx.x, n = gridDim.x;
+  const unsigned full = n & ~7u;
+  if (b >= full) return b;
+  return (long long)(b & 7u) * (full >> 3) + (b >> 3);
+}
+
+// select one of five compile-time node values by a lane-varying index
+__device__ __forceinline__ double sel5(int q, double a0, double a1, double a2, double a3, double a4) {
+  double r = a0;
+  r = q == 1 ? a1 : r;
+  r = q == 2 ? a2 : r;
+  r = q == 3 ? a3 : r;
+  r = q == 4 ? a4 : r;
+  return r;
+}
+
+// trace of all 8 variables of one cell on one side at node q
+template <int SIDE>
+__device__ __forceinline__ void trace_at(const double* __restrict__ U, long long plane, long long c, int q,
+                                         double out[8]) {
+  constexpr double L0 = lam(0), L1 = lam(1), L2 = lam(2), L3 = lam(3), L4 = lam(4);
+  const double s = sel5(q, L0, L1, L2, L3, L4);
+  const double p2 = sel5(q, P2(L0), P2(L1), P2(L2), P2(L3), P2(L4));
+  const double p3 = sel5(q, P3(L0), P3(L1), P3(L2), P3(L3), P3(L4));
+  const double t23 = sel5(q, T23(L0), T23(L1), T23(L2), T23(L3), T23(L4));
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int b = 0; b < 10; b++) {
+      const int p = PL(b, n);
+      if (p >= 0) acc += U[p * plane + c] * phiTr(SIDE, b, s, p2, p3, t23);
+    }
+    out[n] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x faces: Lh.f90:72-136 (traces, fluxes), :206-234 (Riemann),
+// LhEz.f90:6-16 (face EMF).  Face i (0..Nx) of row j (1..Ny) lies between
+// cells i and i+1.
+template <int FT>
+__global__ __launch_bounds__(BS) void k_xface(const double* __restrict__ U, double* __restrict__ FX,
+                                              double* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
+  const long long t = xcd_block() * BS + threadIdx.x;
+  const long long nf = (long long)(g.Nx + 1) * g.Ny;
+  if (t >= nf * 5) return;
+  const int q = (int)(t % 5);
+  const long long f = t / 5;
+  const int i = (int)(f % (g.Nx + 1)), j = 1 + (int)(f / (g.Nx + 1));
+  double UL[8], UR[8], FL[8], FR[8], F[8];
+  trace_at<0>(U, g.plane, cix(g, i, j), q, UL);       // right trace of cell i
+  trace_at<1>(U, g.plane, cix(g, i + 1, j), q, UR);   // left trace of cell i+1
+  lh_flux<double, 1>(UL, FL);
+  lh_flux<double, 1>(UR, FR);
+  double a1, b1, a2, b2;
+  eigenvalueMm<double, 1>(UL, a1, b1);
+  eigenvalueMm<double, 1>(UR, a2, b2);
+  const double SR = tmax(a1, a2), SL = tmin(b1, b2);
+  riemann<double, 1, FT>(SR, SL, UL, UR, FL, FR, F);
+  const long long o = cix(g, i, j);
+#pragma unroll
+  for (int n = 0; n < 8; n++) FX[(q * 8 + n) * g.plane + o] = F[n];
+  const double S = tmax(fabs(UL[1] / UL[0]), fabs(UR[1] / UR[0]));
+  EZ[q * g.plane + o] = -0.5 * (FL[6] + FR[6] - S * (UR[6] - UL[6]));
+}
+
+// y faces: Lh.f90:139-203, :237-264; LhEz.f90:18-28.  Face j (0..Ny) of
+// column i (1..Nx) lies between cells j and j+1.
+template <int FT>
+__global__ __launch_bounds__(BS) void k_yface(const double* __restrict__ U, double* __restrict__ FY,
+                                              double* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
+  const long long t = xcd_block() * BS + threadIdx.x;
+  const long long nf = (long long)g.Nx * (g.Ny + 1);
+  if (t >= nf * 5) return;
+  const int q = (int)(t % 5);
+  const long long f = t / 5;
+  const int i = 1 + (int)(f % g.Nx), j = (int)(f / g.Nx);
+  double UD[8], UU[8], FD[8], FU[8], F[8];
+  trace_at<2>(U, g.plane, cix(g, i, j), q, UD);       // upper trace of cell j (reference UU)
+  trace_at<3>(U, g.plane, cix(g, i, j + 1), q, UU);   // lower trace of cell j+1 (reference UD)
+  lh_flux<double, 2, false>(UD, FD);
+  lh_flux<double, 2, true>(UU, FU);
+  double a1, b1, a2, b2;
+  eigenvalueMm<double, 2>(UD, a1, b1);
+  eigenvalueMm<double, 2>(UU, a2, b2);
+  const double SR = tmax(a1, a2), SL = tmin(b1, b2);
+  riemann<double, 2, FT>(SR, SL, UD, UU, FD, FU, F);
+  const long long o = cix(g, i, j);
+#pragma unroll
+  for (int n = 0; n < 8; n++) FY[(q * 8 + n) * g.plane + o] = F[n];
+  const double S = tmax(fabs(UD[2] / UD[0]), fabs(UU[2] / UU[0]));
+  EZ[q * g.plane + o] = 0.5 * (FD[5] + FU[5] - S * (UU[5] - UD[5]));
+}
+
+// vertex (corner) EMF: LhEz.f90:33-69.  Vertex (i, j) is the upper-right
+// corner of cell (i, j), i = 0..Nx, j = 0..Ny.
+template <int V>
+__device__ __forceinline__ void vertex_state(const double* __restrict__ U, long long plane, long long c, double out[8]) {
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int b = 0; b < 10; b++) {
+      const int p = PL(b, n);
+      if (p >= 0) acc += U[p * plane + c] * phiV(V, b);
+    }
+    out[n] = acc;
+  }
+}
+
+template <int FT>
+__global__ __launch_bounds__(BS) void k_vertex(const double* __restrict__ U, double* __restrict__ EZV, Geo g,
+                                               const DevScal* __restrict__ sc) {
+  const long long t = xcd_block() * BS + threadIdx.x;
+  const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
+  if (t >= nv) return;
+  const int i = (int)(t % (g.Nx + 1)), j = (int)(t / (g.Nx + 1));
+  double URU[8], ULU[8], URD[8], ULD[8];
+  vertex_state<3>(U, g.plane, cix(g, i + 1, j + 1), URU);  // LD corner of the upper-right cell
+  vertex_state<2>(U, g.plane, cix(g, i, j + 1), ULU);      // RD corner of the upper-left cell
+  vertex_state<1>(U, g.plane, cix(g, i + 1, j), URD);      // LU corner of the lower-right cell
+  vertex_state<0>(U, g.plane, cix(g, i, j), ULD);          // RU corner of the lower-left cell
+  EZV[cix(g, i, j)] = vertex_emf<double, FT>(URU, ULU, URD, ULD);
+}
+
+// ---------------------------------------------------------------------------
+// cell residual (Lh.f90:11-68 volume, :268-304 surface + mass inverse) and
+// the RK3 combination of RK3.f90 as the epilogue:
+//   MODE 0: du                     (Lh alone)
+//   MODE 1: out = S + dt du        (uI,  RK3.f90:157)
+//   MODE 2: out = 3/4 U0 + 1/4 (UI + dt du)   (RK3.f90:183,187)
+//   MODE 3: out = 1/3 U0 + 2/3 (UII + dt du)  (RK3.f90:208-210)
+// The launch covers the halo ring too: there du = 0 (Lh.f90:5), so the
+// combination carries the start-of-step halos exactly as the reference's
+// whole-array statements do (the stale halos TVB later reads).
+template <int MODE>
+__global__ __launch_bounds__(BS) void k_cell(const double* __restrict__ S, const double* __restrict__ FX,
+                                             const double* __restrict__ FY, const double* __restrict__ A,
+                                             const double* __restrict__ U0, double* __restrict__ out, Geo g,
+                                             const DevScal* __restrict__ sc, double dt_arg) {
+  const long long t = xcd_block() * BS + threadIdx.x;
+  const int W = g.Nx + 2;
+  if (t >= (long long)W * (g.Ny + 2)) return;
+  const int i = (int)(t % W), j = (int)(t / W);
+  const long long c = cix(g, i, j);
+  const long long pl = g.plane;
+  const double dt = sc ? sc->dt : dt_arg;
+  const bool interior = i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny;
+  double acc[NPL];
+#pragma unroll
+  for (int p = 0; p < NPL; p++) acc[p] = 0.0;
+  if (interior) {
+    double m[NPL];
+#pragma unroll
+    for (int p = 0; p < NPL; p++) m[p] = S[p * pl + c];
+    const double ihx1 = 1.0 / g.hx1, ihy1 = 1.0 / g.hy1;
+    // volume integral over the 5 x 5 Gauss points
+#pragma unroll
+    for (int i1 = 0; i1 < 5; i1++) {
+#pragma unroll
+      for (int j1 = 0; j1 < 5; j1++) {
+        const double x = lam(i1), y = lam(j1);
+        double Uq[8], Fx[8], Fy[8];
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+          double a = 0.0;
+#pragma unroll
+          for (int b = 0; b < 10; b++) {
+            const int p = PL(b, n);
+            if (p >= 0) a += m[p] * phi(b, x, y);
+          }
+          Uq[n] = a;
+        }
+        lh_flux_xy<double>(Uq, Fx, Fy);
+#pragma unroll
+        for (int n = 0; n < 8; n++) { Fx[n] *= ihx1; Fy[n] *= ihy1; }
+        const double ww = 0.25 * wgt(i1) * wgt(j1);
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+#pragma unroll
+          for (int b = 1; b < 10; b++) {
+            const int p = PL(b, n);
+            if (p < 0) continue;
+            const double cx = ww * dphix(b, x, y), cy = ww * dphiy(b, x, y);
+            if (cx != 0.0) acc[p] += cx * Fx[n];
+            if (cy != 0.0) acc[p] += cy * Fy[n];
+          }
+        }
+      }
+    }
+    // surface integrals
+    const double sx = 0.5 / g.hx, sy = 0.5 / g.hy;
+    const long long cL = cix(g, i - 1, j), cD = cix(g, i, j - 1);
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      const double s = lam(q), p2 = P2(s), p3 = P3(s), t23 = T23(s);
+      const double wq = wgt(q);
+#pragma unroll
+      for (int n = 0; n < 8; n++) {
+        const double fr = FX[(q * 8 + n) * pl + c], fl = FX[(q * 8 + n) * pl + cL];
+        const double fu = FY[(q * 8 + n) * pl + c], fd = FY[(q * 8 + n) * pl + cD];
+#pragma unroll
+        for (int b = 0; b < 10; b++) {
+          const int p = PL(b, n);
+          if (p < 0) continue;
+          acc[p] -= sx * wq * (fr * phiTr(0, b, s, p2, p3, t23) - fl * phiTr(1, b, s, p2, p3, t23));
+          acc[p] -= sy * wq * (fu * phiTr(2, b, s, p2, p3, t23) - fd * phiTr(3, b, s, p2, p3, t23));
+        }
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < 8; n++)
+#pragma unroll
+      for (int b = 0; b < 10; b++) {
+        const int p = PL(b, n);
+        if (p >= 0) acc[p] *= (1.0 / mmass(b));
+      }
+  }
+  // epilogue
+#pragma unroll
+  for (int p = 0; p < NPL; p++) {
+    const long long o = p * pl + c;
+    double r;
+    if (MODE == 0) r = acc[p];
+    else if (MODE == 1) r = S[o] + dt * acc[p];
+    else if (MODE == 2) r = (3.0 / 4.0) * U0[o] + (1.0 / 4.0) * (A[o] + dt * acc[p]);
+    else r = (1.0 / 3.0) * U0[o] + (2.0 / 3.0) * (A[o] + dt * acc[p]);
+    out[o] = r;
+  }
+}
+
+// constrained transport: LhEz.f90:71-96 (+ the face combinations of
+// RK3.f90:159-212 for MODE 1..3 as in k_cell).  Face planes: 0..2 Bx modes
+// on x faces (i = 0..Nx, j = 1..Ny), 3..5 By modes on y faces (i = 1..Nx,
+// j = 0..Ny).
+template <int MODE>
+__global__ __launch_bounds__(BS) void k_ct(const double* __restrict__ EZX, const double* __restrict__ EZY,
+                                           const double* __restrict__ EZV, const double* __restrict__ F,
+                                           const double* __restrict__ A, const double* __restrict__ F0,
+                                           double* __restrict__ out, Geo g, const DevScal* __restrict__ sc,
+                                           double dt_arg) {
+  const long long t = xcd_block() * BS + threadIdx.x;
+  const long long nx = (long long)(g.Nx + 1) * g.Ny, ny = (long long)g.Nx * (g.Ny + 1);
+  if (t >= nx + ny) return;
+  const double dt = sc ? sc->dt : dt_arg;
+  const long long pl = g.plane;
+  const double mmE[3] = {1.0, 1.0 / 3.0, 4.0 / 45.0};
+  const double EzP[3] = {1.0, 1.0, 2.0 / 3.0};    // EzR = EzU
+  const double EzM[3] = {1.0, -1.0, 2.0 / 3.0};   // EzL = EzD
+  double d[3];
+  long long o;
+  int base;
+  if (t < nx) {
+    const int i = (int)(t % (g.Nx + 1)), j = 1 + (int)(t / (g.Nx + 1));
+    o = cix(g, i, j);
+    base = 0;
+    const double iy1 = 1.0 / g.hy1;
+    double a[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      const double e = 0.5 * wgt(q) * EZX[q * pl + o];
+      a[1] += e * iy1;
+      a[2] += e * (2 * lam(q) * iy1);
+    }
+    const double vU = EZV[cix(g, i, j)], vD = EZV[cix(g, i, j - 1)];
+#pragma unroll
+    for (int k = 0; k < 3; k++) d[k] = (a[k] - vU * EzP[k] / g.hy + vD * EzM[k] / g.hy) / mmE[k];
+  } else {
+    const long long u = t - nx;
+    const int i = 1 + (int)(u % g.Nx), j = (int)(u / g.Nx);
+    o = cix(g, i, j);
+    base = 3;
+    const double ix1 = 1.0 / g.hx1;
+    double a[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      const double e = 0.5 * wgt(q) * EZY[q * pl + o];
+      a[1] -= e * ix1;
+      a[2] -= e * (2 * lam(q) * ix1);
+    }
+    const double vR = EZV[cix(g, i, j)], vL = EZV[cix(g, i - 1, j)];
+#pragma unroll
+    for (int k = 0; k < 3; k++) d[k] = (a[k] + vR * EzP[k] / g.hx - vL * EzM[k] / g.hx) / mmE[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const long long oo = (base + k) * pl + o;
+    double r;
+    if (MODE == 0) r = d[k];
+    else if (MODE == 1) r = F[oo] + dt * d[k];
+    else if (MODE == 2) r = (3.0 / 4.0) * F0[oo] + (1.0 / 4.0) * (A[oo] + dt * d[k]);
+    else r = (1.0 / 3.0) * F0[oo] + (2.0 / 3.0) * (A[oo] + dt * d[k]);
+    out[oo] = r;
+  }
+}
+
+// whole-array combinations of RK3.f90 for the per-subroutine API
+// (mhd2d_rk_stage): out = a*X + b*(Y + dt*Z), elementwise
+__global__ __launch_bounds__(BS) void k_axpy(double* __restrict__ out, const double* __restrict__ X, double a,
+                                             const double* __restrict__ Y, double b, const double* __restrict__ Z,
+                                             double dt, long long n) {
+  const long long k = (long long)blockIdx.x * BS + threadIdx.x;
+  if (k >= n) return;
+  const double y = Y[k] + dt * Z[k];
+  out[k] = X ? a * X[k] + b * y : y;
+}
+
+// ---------------------------------------------------------------------------
+// positivity limiter of one cell (pp_Limiter.f90:10-81) on modes m[]
+__device__ __forceinline__ double glleval(const double* m, int n, int i1, int j1, int fam) {
+  const double x = fam == 0 ? lamL(i1) : lam(i1);
+  const double y = fam == 0 ? lam(j1) : lamL(j1);
+  double a = 0.0;
+#pragma unroll
+  for (int b = 0; b < 10; b++) {
+    const int p = PL(b, n);
+    if (p >= 0) a += m[p] * phi(b, x, y);
+  }
+  return a;
+}
+
+__device__ __forceinline__ void pp_cell(double m[NPL]) {
+  const double eps = 1e-13;
+  // density
+  double eta = 1.0;
+  const double rbar = m[PL(0, 0)];
+#pragma unroll
+  for (int i1 = 0; i1 < 5; i1++)
+#pragma unroll
+    for (int j1 = 0; j1 < 5; j1++)
+#pragma unroll
+      for (int f = 0; f < 2; f++) {
+        const double rq = glleval(m, 0, i1, j1, f);
+        const double eta1 = fabs((rbar - eps) / (rbar - rq));
+        if (eta1 < 1.0) eta = eta1;
+      }
+  if (eta < 1.0) {
+#pragma unroll
+    for (int b = 1; b < 6; b++) m[PL(b, 0)] = 0.5 * eta * m[PL(b, 0)];
+  }
+  // pressure
+  eta = 1.0;
+  double eta1 = 1.0;
+  double ubar[8];
+#pragma unroll
+  for (int n = 0; n < 8; n++) ubar[n] = m[PL(0, n)];
+#pragma unroll 1
+  for (int k = 0; k < 50; k++) {
+    const int i1 = k / 10, j1 = (k / 2) % 5, f = k % 2;
+    double uq[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      const double x = f == 0 ? lamL(i1) : lam(i1);
+      const double y = f == 0 ? lam(j1) : lamL(j1);
+      double a = 0.0;
+#pragma unroll
+      for (int b = 0; b < 10; b++) {
+        const int p = PL(b, n);
+        if (p < 0) continue;
+        // basis values at run-time nodes: same formulas as the tables
+        double v;
+        {
+          MHD_NOCONTRACT
+          v = b == 0 ? 1.0 : b == 1 ? x : b == 2 ? y : b == 3 ? x * x - 1.0 / 3.0 : b == 4 ? x * y
+            : b == 5 ? y * y - 1.0 / 3.0 : b == 6 ? x * x * x - 3.0 * x / 5.0 : b == 7 ? (x * x - 1.0 / 3.0) * (y)
+            : b == 8 ? (x) * (y * y - 1.0 / 3.0) : y * y * y - 3.0 * y / 5.0;
+        }
+        a += m[p] * v;
+      }
+      uq[n] = a;
+    }
+    const double pq = pressure<double>(uq);
+    if (pq < 0.0) {
+      // calculate_tq (pp_Limiter.f90:101-125): bisection, at most 47 halvings
+      double ta = 0.0, tb = 1.0, ut[8];
+      while (tb - ta > 1e-14) {
+        const double tq = 0.5 * (ta + tb);
+#pragma unroll
+        for (int n = 0; n < 8; n++) ut[n] = tq * uq[n] + (1 - tq) * ubar[n];
+        if (pressure<double>(ut) < 1e-12) tb = tq;
+        else ta = tq;
+      }
+      const double tq = ta;
+      double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+      for (int n = 0; n < 8; n++) {
+        const double sq = tq * uq[n] + (1 - tq) * ubar[n];
+        s1 += (sq - ubar[n]) * (sq - ubar[n]);
+        s2 += (uq[n] - ubar[n]) * (uq[n] - ubar[n]);
+      }
+      eta1 = sqrt(s1) / sqrt(s2);
+    }
+    if (eta1 < eta) eta = eta1;
+  }
+  if (eta < 1.0) eta = 0.5 * eta;
+#pragma unroll
+  for (int n = 0; n < 8; n++)
+#pragma unroll
+    for (int b = 1; b < 10; b++) {
+      const int p = PL(b, n);
+      if (p >= 0) m[p] = eta * m[p];
+    }
+}
+
+// div_free_Balsara.f90:15-78: in-cell B from the four faces
+__device__ __forceinline__ void divfree_cell(double m[NPL], const double* __restrict__ F, const Geo& g, int i, int j) {
+  const long long pl = g.plane;
+  const long long cR = cix(g, i, j), cL = cix(g, i - 1, j), cD = cix(g, i, j - 1);
+  const double a0R = F[0 * pl + cR], a1R = F[1 * pl + cR], a2R = F[2 * pl + cR];
+  const double a0L = F[0 * pl + cL], a1L = F[1 * pl + cL], a2L = F[2 * pl + cL];
+  const double b0U = F[3 * pl + cR], b1U = F[4 * pl + cR], b2U = F[5 * pl + cR];
+  const double b0D = F[3 * pl + cD], b1D = F[4 * pl + cD], b2D = F[5 * pl + cD];
+  const double rxy = g.hx / g.hy, ryx = g.hy / g.hx;
+  m[PL(0, 5)] = 0.5 * (a0R + a0L) + (1.0 / 6.0) * rxy * (b1U - b1D);
+  m[PL(1, 5)] = 0.5 * (a0R - a0L) + (1.0 / 15.0) * rxy * (b2U - b2D);
+  m[PL(2, 5)] = 0.5 * (a1R + a1L);
+  m[PL(3, 5)] = -(1.0 / 4.0) * rxy * (b1U - b1D);
+  m[PL(4, 5)] = 0.5 * (a1R - a1L);
+  m[PL(5, 5)] = 0.5 * (a2R + a2L);
+  m[PL(6, 5)] = -(1.0 / 6.0) * rxy * (b2U - b2D);
+  m[PL(8, 5)] = 0.5 * (a2R - a2L);
+  m[PL(0, 6)] = 0.5 * (b0U + b0D) + (1.0 / 6.0) * ryx * (a1R - a1L);
+  m[PL(2, 6)] = 0.5 * (b0U - b0D) + (1.0 / 15.0) * ryx * (a2R - a2L);
+  m[PL(1, 6)] = 0.5 * (b1U + b1D);
+  m[PL(5, 6)] = -(1.0 / 4.0) * ryx * (a1R - a1L);
+  m[PL(4, 6)] = 0.5 * (b1U - b1D);
+  m[PL(3, 6)] = 0.5 * (b2U + b2D);
+  m[PL(9, 6)] = -(1.0 / 6.0) * ryx * (a2R - a2L);
+  m[PL(7, 6)] = 0.5 * (b2U - b2D);
+}
+
+__device__ __forceinline__ bool is_bplane(int p) { return (p >= 30 && p < 42) || p >= 48; }
+
+// limiter pass (TVB_Limiter_P2.f90:16-121 + pp + div_free_Balsara) over
+// the whole array: interior cells are limited, halo cells copied (or, with
+// DIV, their B planes zeroed as div_free_Balsara.f90:13 does).
+// flags: indicator == 2/3 reads trouble[] (1 = limit).
+template <bool TVB, bool PP, bool DIV>
+__global__ __launch_bounds__(BS) void k_limit(const double* __restrict__ in, double* __restrict__ out,
+                                              const double* __restrict__ F, const int* __restrict__ trouble,
+                                              Geo g, double M, const DevScal* __restrict__ sc) {
+  const long long t = xcd_block() * BS + threadIdx.x;
+  const int W = g.Nx + 2;
+  if (t >= (long long)W * (g.Ny + 2)) return;
+  const int i = (int)(t % W), j = (int)(t / W);
+  const long long c = cix(g, i, j), pl = g.plane;
+  const bool interior = i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny;
+  if (!interior) {
+#pragma unroll 4
+    for (int p = 0; p < NPL; p++) out[p * pl + c] = (DIV && is_bplane(p)) ? 0.0 : in[p * pl + c];
+    return;
+  }
+  double m[NPL];
+#pragma unroll
+  for (int p = 0; p < NPL; p++) m[p] = in[p * pl + c];
+  if (TVB && (!trouble || trouble[(long long)(j - 1) * g.Nx + (i - 1)] == 1)) {
+    double U[8], dR[8], dL[8], r1[8], l1[8], Rm[8], Lm[8], Um[8], Dm[8];
+    int change = 0;
+    const long long cE = cix(g, i + 1, j), cW = cix(g, i - 1, j), cN = cix(g, i, j + 1), cS = cix(g, i, j - 1);
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      U[n] = m[PL(0, n)];
+      dR[n] = in[PL(0, n) * pl + cE] - U[n];
+      dL[n] = U[n] - in[PL(0, n) * pl + cW];
+      r1[n] = m[PL(1, n)] + (2.0 / 3.0) * m[PL(3, n)];
+      l1[n] = m[PL(1, n)] - (2.0 / 3.0) * m[PL(3, n)];
+    }
+    tvb_dir<1>(U, dR, dL, r1, l1, M * (g.hx * g.hx), Rm, Lm);
+#pragma unroll
+    for (int n = 0; n < 8; n++)
+      if (fabs(Rm[n] - r1[n]) > 1e-6 || fabs(Lm[n] - l1[n]) > 1e-6) change |= 1 << n;
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      dR[n] = in[PL(0, n) * pl + cN] - U[n];
+      dL[n] = U[n] - in[PL(0, n) * pl + cS];
+      r1[n] = m[PL(2, n)] + (2.0 / 3.0) * m[PL(5, n)];
+      l1[n] = m[PL(2, n)] - (2.0 / 3.0) * m[PL(5, n)];
+    }
+    tvb_dir<2>(U, dR, dL, r1, l1, M * (g.hy * g.hy), Um, Dm);
+#pragma unroll
+    for (int n = 0; n < 8; n++)
+      if (fabs(Um[n] - r1[n]) > 1e-6 || fabs(Dm[n] - l1[n]) > 1e-6) change |= 1 << n;
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      if (!(change & (1 << n))) continue;
+#pragma unroll
+      for (int b = 6; b < 10; b++) {
+        const int p = PL(b, n);
+        if (p >= 0) m[p] = 0.0;
+      }
+      m[PL(4, n)] = 0.0;
+      m[PL(1, n)] = 0.5 * (Rm[n] + Lm[n]);
+      m[PL(2, n)] = 0.5 * (Um[n] + Dm[n]);
+      m[PL(3, n)] = (3.0 / 4.0) * (Rm[n] - Lm[n]);
+      m[PL(5, n)] = (3.0 / 4.0) * (Um[n] - Dm[n]);
+    }
+  }
+  if (PP) pp_cell(m);
+  if (DIV) divfree_cell(m, F, g, i, j);
+#pragma unroll
+  for (int p = 0; p < NPL; p++) out[p * pl + c] = m[p];
+}
+
+// set_bc.f90:5-27 over the halo ring: periodic (1) or outflow (2); the
+// x-then-y order of the reference makes the corners (map_x(i), map_y(j)).
+__global__ __launch_bounds__(BS) void k_setbc(double* __restrict__ U, Geo g, int nplanes,
+                                              const DevScal* __restrict__ sc) {
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  const int W = g.Nx + 2, H = g.Ny + 2;
+  const long long ring = 2LL * W + 2LL * g.Ny;
+  if (t >= ring * nplanes) return;
+  const int p = (int)(t / ring);
+  const long long r = t % ring;
+  int i, j;
+  if (r < W) { i = (int)r; j = 0; }
+  else if (r < 2 * W) { i = (int)(r - W); j = H - 1; }
+  else if (r < 2 * W + g.Ny) { i = 0; j = 1 + (int)(r - 2 * W); }
+  else { i = W - 1; j = 1 + (int)(r - 2 * W - g.Ny); }
+  auto mapx = [&](int a) {
+    if (a == 0) return g.bcL == 1 ? g.Nx : g.bcL == 2 ? 1 : 0;
+    if (a == g.Nx + 1) return g.bcR == 1 ? 1 : g.bcR == 2 ? g.Nx : g.Nx + 1;
+    return a;
+  };
+  auto mapy = [&](int b) {
+    if (b == 0) return g.bcD == 1 ? g.Ny : g.bcD == 2 ? 1 : 0;
+    if (b == g.Ny + 1) return g.bcU == 1 ? 1 : g.bcU == 2 ? g.Ny : g.Ny + 1;
+    return b;
+  };
+  const int si = mapx(i), sj = mapy(j);
+  if (si == i && sj == j) return;
+  U[p * g.plane + cix(g, i, j)] = U[p * g.plane + cix(g, si, sj)];
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_down(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+__global__ void k_set_tend(DevScal* sc, double tend) {
+  if (threadIdx.x == 0) sc->tend = tend;
+}
+
+__global__ void k_scal_reset(DevScal* sc, int which) {
+  if (threadIdx.x != 0) return;
+  if (which == 0) { sc->ax_bits = 0ull; sc->ay_bits = 0ull; }
+  else { sc->umax_bits = 0ull; sc->pmin_key = dkey(1000000.0); sc->nflag = 0; }
+}
+
+// calculate_dt.f90:12-24: max |u_n +- c_f| at the cell averages
+__global__ __launch_bounds__(BS) void k_dt(const double* __restrict__ U, Geo g, DevScal* sc) {
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  double ax = 0.0, ay = 0.0;
+  if (t < (long long)g.Nx * g.Ny) {
+    const int i = 1 + (int)(t % g.Nx), j = 1 + (int)(t / g.Nx);
+    double u[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) u[n] = U[PL(0, n) * g.plane + cix(g, i, j)];
+    double a1, a2;
+    eigenvalueMm<double, 1>(u, a1, a2);
+    const double mx = fmax(fabs(a1), fabs(a2));
+    if (mx == mx) ax = mx;     // NaN never exceeds the running max
+    eigenvalueMm<double, 2>(u, a1, a2);
+    const double my = fmax(fabs(a1), fabs(a2));
+    if (my == my) ay = my;
+  }
+  __shared__ double sx[BS / 64], sy[BS / 64];
+  ax = wave_max(ax);
+  ay = wave_max(ay);
+  if ((threadIdx.x & 63) == 0) { sx[threadIdx.x >> 6] = ax; sy[threadIdx.x >> 6] = ay; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BS / 64; w++) { ax = fmax(ax, sx[w]); ay = fmax(ay, sy[w]); }
+    atomicMax(&sc->ax_bits, (unsigned long long)__double_as_longlong(ax));
+    atomicMax(&sc->ay_bits, (unsigned long long)__double_as_longlong(ay));
+  }
+}
+
+// RK3.f90:137-148: dt from the maxima, clipped to tend; steps that start at
+// t >= tend are switched off (active = 0) like the reference's loop test.
+__global__ void k_dt_final(DevScal* sc, double hx, double hy, int advance) {
+  if (threadIdx.x != 0) return;
+  const double ax = __longlong_as_double((long long)sc->ax_bits);
+  const double ay = __longlong_as_double((long long)sc->ay_bits);
+  sc->alphax = ax;
+  sc->alphay = ay;
+  double dt = sc->CFL / (ax / hx + ay / hy);
+  if (!advance) { sc->dt = dt; return; }
+  if (!(sc->t < sc->tend)) { sc->active = 0; return; }
+  sc->active = 1;
+  if (sc->t + dt >= sc->tend) { dt = sc->tend - sc->t; sc->t = sc->tend; }
+  else sc->t = sc->t + dt;
+  if (dt < 1e-7) sc->t = sc->tend;
+  sc->dt = dt;
+  sc->count += 1;
+}
+
+// calculate_umax.f90, calculate_pmin.f90 and per-block partial sums of
+// calculate_totaldiv.f90 (summed in a fixed order by k_div_final)
+__global__ __launch_bounds__(BS) void k_diag(const double* __restrict__ U, Geo g, DevScal* sc, double* partial) {
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  double um = 0.0, pm = 1000000.0, dv = 0.0;
+  if (t < (long long)g.Nx * g.Ny) {
+    const int i = 1 + (int)(t % g.Nx), j = 1 + (int)(t / g.Nx);
+    const long long c = cix(g, i, j), pl = g.plane;
+    double m[NPL];
+#pragma unroll
+    for (int p = 0; p < NPL; p++) m[p] = U[p * pl + c];
+    um = fabs(m[PL(0, 0)]);
+    const double ihx1 = 1.0 / g.hx1, ihy1 = 1.0 / g.hy1;
+#pragma unroll
+    for (int i1 = 0; i1 < 5; i1++)
+#pragma unroll
+      for (int j1 = 0; j1 < 5; j1++) {
+        const double x = lam(i1), y = lam(j1);
+        double uq[8];
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+          double a = 0.0;
+#pragma unroll
+          for (int b = 0; b < 10; b++) {
+            const int p = PL(b, n);
+            if (p >= 0) a += m[p] * phi(b, x, y);
+          }
+          uq[n] = a;
+        }
+        const double p1 = pressure<double>(uq);
+        if (p1 < pm) pm = p1;
+        double dvg = 0.0;
+#pragma unroll
+        for (int b = 0; b < 10; b++) {
+          const int p6 = PL(b, 5), p7 = PL(b, 6);
+          if (p6 >= 0 && dphix(b, x, y) != 0.0) dvg += m[p6] * (dphix(b, x, y) * ihx1);
+          if (p7 >= 0 && dphiy(b, x, y) != 0.0) dvg += m[p7] * (dphiy(b, x, y) * ihy1);
+        }
+        dv += wgt(i1) * wgt(j1) * (dvg * dvg);
+      }
+  }
+  __shared__ double s1[BS / 64], s2[BS / 64], s3[BS / 64];
+  um = wave_max(um);
+  pm = wave_min(pm);
+  dv = wave_sum(dv);
+  if ((threadIdx.x & 63) == 0) { s1[threadIdx.x >> 6] = um; s2[threadIdx.x >> 6] = pm; s3[threadIdx.x >> 6] = dv; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BS / 64; w++) { um = fmax(um, s1[w]); pm = fmin(pm, s2[w]); dv += s3[w]; }
+    atomicMax(&sc->umax_bits, (unsigned long long)__double_as_longlong(um));
+    atomicMin(&sc->pmin_key, dkey(pm));
+    partial[blockIdx.x] = dv;
+  }
+}
+
+__global__ void k_div_final(DevScal* sc, const double* partial, long long nblocks, double hx1, double hy1) {
+  __shared__ double s[BS];
+  double a = 0.0;
+  for (long long k = threadIdx.x; k < nblocks; k += BS) a += partial[k];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int k = 0; k < BS; k++) tot += s[k];
+    sc->totaldiv = sqrt(tot * hx1 * hy1);
+    sc->umax = __longlong_as_double((long long)sc->umax_bits);
+    sc->pmin = dunkey(sc->pmin_key);
+  }
+}
+
+// KXRCF_Detector.f90:180-231 (uG per cell, flags reset each call, Ck given)
+template <int SIDE>
+__device__ __forceinline__ double face_jump_int(const double* __restrict__ U, long long pl, long long cself,
+                                                long long cnb, int n) {
+  // sum_q w_q (u_self(q) - u_nb(q)) on the shared face
+  constexpr int OPP = SIDE == 0 ? 1 : SIDE == 1 ? 0 : SIDE == 2 ? 3 : 2;
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const double x = lam(q), p2 = P2(x), p3 = P3(x), t23 = T23(x);
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int bb = 0; bb < 10; bb++) {
+      const int p = PL(bb, n);
+      if (p < 0) continue;
+      a += U[p * pl + cself] * phiTr(SIDE, bb, x, p2, p3, t23);
+      b += U[p * pl + cnb] * phiTr(OPP, bb, x, p2, p3, t23);
+    }
+    s += wgt(q) * (a - b);
+  }
+  return s;
+}
+
+__global__ __launch_bounds__(BS) void k_kxrcf(const double* __restrict__ U, int* __restrict__ trouble, Geo g,
+                                              double Ck, DevScal* sc) {
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  if (t >= (long long)g.Nx * g.Ny) return;
+  const int i = 1 + (int)(t % g.Nx), j = 1 + (int)(t / g.Nx);
+  const long long pl = g.plane, c = cix(g, i, j);
+  const long long cE = cix(g, i + 1, j), cW = cix(g, i - 1, j), cN = cix(g, i, j + 1), cS = cix(g, i, j - 1);
+  const bool inE = U[PL(0, 1) * pl + cE] < 0.0, inW = U[PL(0, 1) * pl + cW] > 0.0;
+  const bool inN = U[PL(0, 2) * pl + cN] < 0.0, inS = U[PL(0, 2) * pl + cS] > 0.0;
+  double Lij = 1e-12;
+  if (inE) Lij += 5 * g.hy;
+  if (inW) Lij += 5 * g.hy;
+  if (inN) Lij += 5 * g.hx;
+  if (inS) Lij += 5 * g.hx;
+  const double hk = pow(g.hx * g.hy, 0.25 * (2 + 1));
+  int flag = 0;
+  for (int n = 0; n < 8; n++) {
+    double I = 0.0;
+    if (inE) I += g.hy1 * face_jump_int<0>(U, pl, c, cE, n);
+    if (inW) I += g.hy1 * face_jump_int<1>(U, pl, c, cW, n);
+    if (inN) I += g.hx1 * face_jump_int<2>(U, pl, c, cN, n);
+    if (inS) I += g.hx1 * face_jump_int<3>(U, pl, c, cS, n);
+    double uinf = 0.0;
+#pragma unroll
+    for (int i1 = 0; i1 < 5; i1++)
+#pragma unroll
+      for (int j1 = 0; j1 < 5; j1++) {
+        double a = 0.0;
+#pragma unroll
+        for (int b = 0; b < 10; b++) {
+          const int p = PL(b, n);
+          if (p >= 0) a += U[p * pl + c] * phi(b, lam(i1), lam(j1));
+        }
+        uinf = fmax(uinf, fabs(a) + 1e-12);
+      }
+    if (fabs(I) / (hk * Lij * uinf) >= Ck) flag = 1;
+  }
+  trouble[t] = flag;
+  if (sc && flag) atomicAdd((unsigned long long*)&sc->nflag, 1ull);
+}
+
+// ---------------------------------------------------------------------------
+// initial conditions (init_data.f90:21-121; init1/3/4.txt; Rotor_functions.f90)
+struct ICPar {
+  int ic;
+  int pad;
+  double par[32];
+  double kx[16], ky[16], phu[16], phv[16];
+};
+
+__device__ void ic_eval(const ICPar& c, double x, double y, double U[8]) {
+  const double pi = 4 * atan(1.0);
+  double rho = 1, v1 = 0, v2 = 0, v3 = 0, B1 = 0, B2 = 0, B3 = 0, p = 1;
+  const double gam = 5.0 / 3.0;
+  if (c.ic == MHD2D_IC_ORSZAG_TANG || c.ic == MHD2D_IC_OT_PERTURBED) {
+    p = gam;
+    rho = gam * gam;
+    v1 = -sin(y);
+    v2 = sin(x);
+    B1 = -sin(y);
+    B2 = sin(2.0 * x);
+    if (c.ic == MHD2D_IC_OT_PERTURBED) {
+      const double amp = c.par[0] / 16.0;
+      double du = 0, dv = 0;
+      for (int m = 0; m < 16; m++) {
+        du += sin(c.kx[m] * x + c.ky[m] * y + c.phu[m]);
+        dv += sin(c.kx[m] * x + c.ky[m] * y + c.phv[m]);
+      }
+      v1 += amp * du;
+      v2 += amp * dv;
+    }
+  } else if (c.ic == MHD2D_IC_VORTEX) {
+    const double r2 = x * x + y * y;
+    p = 1 - r2 / (8.0 * (pi * pi)) * exp(1 - r2);
+    rho = 1;
+    v1 = 1 - 1.0 / (2.0 * pi) * y * exp(0.5 * (1 - r2));
+    v2 = 1 + 1.0 / (2.0 * pi) * x * exp(0.5 * (1 - r2));
+    B1 = -1.0 / (2.0 * pi) * y * exp(0.5 * (1 - r2));
+    B2 = 1.0 / (2.0 * pi) * x * exp(0.5 * (1 - r2));
+  } else if (c.ic == MHD2D_IC_ROTOR) {
+    const double r0 = c.par[0], r1 = c.par[1], spin = c.par[2], xc = c.par[6], yc = c.par[7];
+    const double r = sqrt((x - xc) * (x - xc) + (y - yc) * (y - yc));
+    const double f = (r1 - r) / (r1 - r0);
+    const double rin = c.par[5];
+    if (r < r0) { rho = rin; v1 = -spin * (y - yc) / r0; v2 = spin * (x - xc) / r0; }
+    else if (r < r1) { rho = 1.0 + (rin - 1.0) * f; v1 = -spin * f * (y - yc) / r; v2 = spin * f * (x - xc) / r; }
+    else { rho = 1.0; v1 = 0.0; v2 = 0.0; }
+    p = c.par[3];
+    B1 = c.par[4];
+  } else if (c.ic == MHD2D_IC_RIEMANN4) {
+    const int q = (x >= c.par[0] ? 0 : 1) + (y >= c.par[1] ? 0 : 2);
+    const double* st = &c.par[2 + 6 * q];
+    rho = st[0]; v1 = st[1]; v2 = st[2]; p = st[3]; B1 = st[4]; B2 = st[5];
+  }
+  U[0] = rho;
+  U[1] = rho * v1;
+  U[2] = rho * v2;
+  U[3] = rho * v3;
+  U[4] = p / (gam - 1.0) + 0.5 * rho * (v1 * v1 + v2 * v2) + 0.5 * (B1 * B1 + B2 * B2);
+  U[5] = B1;
+  U[6] = B2;
+  U[7] = B3;
+}
+
+__global__ __launch_bounds__(BS) void k_init_cells(double* __restrict__ S, Geo g, double xa, double ya,
+                                                   const ICPar* __restrict__ icp) {
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  const int W = g.Nx + 2;
+  if (t >= (long long)W * (g.Ny + 2)) return;
+  const int i = (int)(t % W), j = (int)(t / W);
+  const long long c = cix(g, i, j), pl = g.plane;
+  if (!(i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny)) {
+    for (int p = 0; p < NPL; p++) S[p * pl + c] = 0.0;
+    return;
+  }
+  const ICPar& ic = *icp;
+  const double Xc = xa + (i - 0.5) * g.hx, Yc = ya + (j - 0.5) * g.hy;
+  double acc[8][6];
+  for (int n = 0; n < 8; n++)
+    for (int b = 0; b < 6; b++) acc[n][b] = 0.0;
+  for (int i1 = 0; i1 < 5; i1++)
+    for (int j1 = 0; j1 < 5; j1++) {
+      double U[8];
+      ic_eval(ic, Xc + g.hx1 * lam(i1), Yc + g.hy1 * lam(j1), U);
+      for (int b = 0; b < 6; b++) {
+        const double ww = wgt(i1) * wgt(j1) * phi(b, lam(i1), lam(j1));
+        for (int n = 0; n < 8; n++) acc[n][b] += ww * U[n];
+      }
+    }
+  for (int n = 0; n < 8; n++)
+    for (int b = 0; b < 10; b++) {
+      const int p = PL(b, n);
+      if (p < 0) continue;
+      S[p * pl + c] = b < 6 ? (0.25 * acc[n][b]) / mmass(b) : 0.0;   // init_data.f90:43 leaves modes 7-10 at 0
+    }
+}
+
+__global__ __launch_bounds__(BS) void k_init_faces(double* __restrict__ F, Geo g, double xa, double ya,
+                                                   const ICPar* __restrict__ icp) {
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  const long long nx = (long long)(g.Nx + 1) * g.Ny, ny = (long long)g.Nx * (g.Ny + 1);
+  if (t >= nx + ny) return;
+  const ICPar& ic = *icp;
+  const long long pl = g.plane;
+  const double mmE[3] = {1.0, 1.0 / 3.0, 4.0 / 45.0};
+  if (t < nx) {
+    const int i = (int)(t % (g.Nx + 1)), j = 1 + (int)(t / (g.Nx + 1));
+    const double Yc = ya + (j - 0.5) * g.hy;
+    double a[3] = {0, 0, 0};
+    for (int q = 0; q < 5; q++) {
+      double U[8];
+      ic_eval(ic, xa + i * g.hx, Yc + g.hy1 * lam(q), U);
+      const double e[3] = {1.0, lam(q), P2(lam(q))};
+      for (int k = 0; k < 3; k++) a[k] += 0.5 * wgt(q) * e[k] * U[5];
+    }
+    for (int k = 0; k < 3; k++) F[k * pl + cix(g, i, j)] = a[k] / mmE[k];
+  } else {
+    const long long u = t - nx;
+    const int i = 1 + (int)(u % g.Nx), j = (int)(u / g.Nx);
+    const double Xc = xa + (i - 0.5) * g.hx;
+    double a[3] = {0, 0, 0};
+    for (int q = 0; q < 5; q++) {
+      double U[8];
+      ic_eval(ic, Xc + g.hx1 * lam(q), ya + j * g.hy, U);
+      const double e[3] = {1.0, lam(q), P2(lam(q))};
+      for (int k = 0; k < 3; k++) a[k] += 0.5 * wgt(q) * e[k] * U[6];
+    }
+    for (int k = 0; k < 3; k++) F[(3 + k) * pl + cix(g, i, j)] = a[k] / mmE[k];
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// host side
+
+struct mhd2d_ctx {
+  mhd2d_config cfg;
+  Geo g;
+  hipStream_t stream;
+  int device;
+  // state buffers (52 planes each); roles rotate inside mhd2d_step
+  double *S, *U0, *UI, *UII, *T;
+  double* du;
+  // face buffers (6 planes each)
+  double *F, *F0, *FI, *FII, *Ft, *dEz;
+  // face fluxes and EMFs
+  double *FX, *FY, *EZX, *EZY, *EZV;
+  int* trouble;
+  DevScal* sc;
+  double* partial;
+  ICPar* icp;
+  long long nblk_cells;
+  bool lh_done;
+  char err[512];
+};
+
+namespace {
+
+int fail(mhd2d_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(ctx->err, sizeof ctx->err, fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ctx, -2, "%s failed: %s", #call, hipGetErrorString(e_));             \
+  } while (0)
+#define LAUNCHCHK(ctx)                                                                 \
+  do {                                                                                 \
+    hipError_t e_ = hipGetLastError();                                                 \
+    if (e_ != hipSuccess) return fail(ctx, -3, "kernel launch: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+inline unsigned nblk(long long n) { return (unsigned)((n + BS - 1) / BS); }
+
+template <typename F>
+int dispatch_ft(int ft, F&& f) {
+  switch (ft) {
+    case 1: return f(std::integral_constant<int, 1>{});
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 3: return f(std::integral_constant<int, 3>{});
+    default: return f(std::integral_constant<int, 4>{});
+  }
+}
+
+// face fluxes + face EMFs (+ vertex EMFs when with_vertex)
+int launch_fluxes(mhd2d_ctx* ctx, const double* U, bool with_vertex, const DevScal* sc) {
+  const Geo& g = ctx->g;
+  return dispatch_ft(ctx->cfg.flux_type, [&](auto FTc) -> int {
+    constexpr int FT = decltype(FTc)::value;
+    const long long nxf = (long long)(g.Nx + 1) * g.Ny * 5, nyf = (long long)g.Nx * (g.Ny + 1) * 5;
+    hipLaunchKernelGGL(k_xface<FT>, dim3(nblk(nxf)), dim3(BS), 0, ctx->stream, U, ctx->FX, ctx->EZX, g, sc);
+    hipLaunchKernelGGL(k_yface<FT>, dim3(nblk(nyf)), dim3(BS), 0, ctx->stream, U, ctx->FY, ctx->EZY, g, sc);
+    if (with_vertex) {
+      const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
+      hipLaunchKernelGGL(k_vertex<FT>, dim3(nblk(nv)), dim3(BS), 0, ctx->stream, U, ctx->EZV, g, sc);
+    }
+    LAUNCHCHK(ctx);
+    return 0;
+  });
+}
+
+int launch_cell(mhd2d_ctx* ctx, int mode, const double* S, const double* A, const double* U0, double* out,
+                const DevScal* sc, double dt) {
+  const Geo& g = ctx->g;
+  const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
+  switch (mode) {
+    case 0: hipLaunchKernelGGL(k_cell<0>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, ctx->FX, ctx->FY, A, U0, out, g, sc, dt); break;
+    case 1: hipLaunchKernelGGL(k_cell<1>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, ctx->FX, ctx->FY, A, U0, out, g, sc, dt); break;
+    case 2: hipLaunchKernelGGL(k_cell<2>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, ctx->FX, ctx->FY, A, U0, out, g, sc, dt); break;
+    default: hipLaunchKernelGGL(k_cell<3>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, ctx->FX, ctx->FY, A, U0, out, g, sc, dt); break;
+  }
+  LAUNCHCHK(ctx);
+  return 0;
+}
+
+int launch_ct(mhd2d_ctx* ctx, int mode, const double* F, const double* A, const double* F0, double* out,
+              const DevScal* sc, double dt) {
+  const Geo& g = ctx->g;
+  const long long n = (long long)(g.Nx + 1) * g.Ny + (long long)g.Nx * (g.Ny + 1);
+  switch (mode) {
+    case 0: hipLaunchKernelGGL(k_ct<0>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, ctx->EZX, ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt); break;
+    case 1: hipLaunchKernelGGL(k_ct<1>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, ctx->EZX, ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt); break;
+    case 2: hipLaunchKernelGGL(k_ct<2>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, ctx->EZX, ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt); break;
+    default: hipLaunchKernelGGL(k_ct<3>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, ctx->EZX, ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt); break;
+  }
+  LAUNCHCHK(ctx);
+  return 0;
+}
+
+int launch_setbc(mhd2d_ctx* ctx, double* U, const DevScal* sc) {
+  const Geo& g = ctx->g;
+  const long long ring = 2LL * (g.Nx + 2) + 2LL * g.Ny;
+  hipLaunchKernelGGL(k_setbc, dim3(nblk(ring * NPL)), dim3(BS), 0, ctx->stream, U, g, NPL, sc);
+  LAUNCHCHK(ctx);
+  return 0;
+}
+
+int launch_kxrcf(mhd2d_ctx* ctx, const double* U, DevScal* sc) {
+  const Geo& g = ctx->g;
+  hipLaunchKernelGGL(k_kxrcf, dim3(nblk((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream, U, ctx->trouble, g,
+                     ctx->cfg.Ck, sc);
+  LAUNCHCHK(ctx);
+  return 0;
+}
+
+// limiter pass in -> out (TVB with the configured pp / indicator, then
+// optionally div_free); trouble flags are computed first when needed
+int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool tvb, bool pp, bool div,
+                 DevScal* sc) {
+  const Geo& g = ctx->g;
+  const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
+  const int ind = ctx->cfg.indicator;
+  const int* trouble = nullptr;
+  if (tvb && ind == MHD2D_IND_NONE) tvb = false;
+  if (tvb && (ind == MHD2D_IND_KXRCF || ind == MHD2D_IND_MLP)) {
+    int rc = launch_kxrcf(ctx, in, sc);
+    if (rc) return rc;
+    trouble = ctx->trouble;
+  }
+  const double M = ctx->cfg.M;
+  const dim3 gr(nblk(n)), bl(BS);
+#define LIM(a, b, c) hipLaunchKernelGGL((k_limit<a, b, c>), gr, bl, 0, ctx->stream, in, out, F, trouble, g, M, sc)
+  if (tvb && pp && div) LIM(true, true, true);
+  else if (tvb && pp) LIM(true, true, false);
+  else if (tvb && div) LIM(true, false, true);
+  else if (tvb) LIM(true, false, false);
+  else if (pp && div) LIM(false, true, true);
+  else if (pp) LIM(false, true, false);
+  else if (div) LIM(false, false, true);
+  else LIM(false, false, false);
+#undef LIM
+  LAUNCHCHK(ctx);
+  return 0;
+}
+
+long long geo_plane(const Geo& g) { return (long long)(g.Ny + 2) * g.P; }
+
+int ctx_check(mhd2d_ctx* ctx) {
+  if (!ctx) return -1;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* mhd2d_last_error(const mhd2d_ctx* ctx) { return ctx ? ctx->err : "null context"; }
+
+int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
+  if (!cfg || !out) return -1;
+  *out = nullptr;
+  if (cfg->Nx < 2 || cfg->Ny < 2 || cfg->flux_type < 1 || cfg->flux_type > 4) return -1;
+  if (!(cfg->xb > cfg->xa) || !(cfg->yb > cfg->ya)) return -1;
+  const int bcs[4] = {cfg->bcR, cfg->bcL, cfg->bcU, cfg->bcD};
+  for (int b : bcs)
+    if (b != 1 && b != 2) return -1;
+  if (cfg->precision != MHD2D_FP64) return -4;   // only the plain form is built in this library version
+  mhd2d_ctx* ctx = (mhd2d_ctx*)calloc(1, sizeof(mhd2d_ctx));
+  if (!ctx) return -1;
+  ctx->cfg = *cfg;
+  if (ctx->cfg.CFL <= 0) ctx->cfg.CFL = 0.15;
+  Geo& g = ctx->g;
+  g.Nx = cfg->Nx;
+  g.Ny = cfg->Ny;
+  g.P = ((cfg->Nx + 2 + OFF) + 15) / 16 * 16;
+  g.bcR = cfg->bcR; g.bcL = cfg->bcL; g.bcU = cfg->bcU; g.bcD = cfg->bcD;
+  g.plane = geo_plane(g);
+  g.hx = (cfg->xb - cfg->xa) / cfg->Nx;      // init_data.f90:21-24
+  g.hy = (cfg->yb - cfg->ya) / cfg->Ny;
+  g.hx1 = 0.5 * g.hx;
+  g.hy1 = 0.5 * g.hy;
+  hipError_t e = hipGetDevice(&ctx->device);
+  if (e != hipSuccess) { free(ctx); return -2; }
+  const size_t sbytes = (size_t)NPL * g.plane * sizeof(double);
+  const size_t fbytes = (size_t)6 * g.plane * sizeof(double);
+  const size_t xbytes = (size_t)40 * g.plane * sizeof(double);
+  const size_t ebytes = (size_t)5 * g.plane * sizeof(double);
+  ctx->nblk_cells = nblk((long long)g.Nx * g.Ny);
+  struct { void** p; size_t n; } al[] = {
+      {(void**)&ctx->S, sbytes},   {(void**)&ctx->U0, sbytes},  {(void**)&ctx->UI, sbytes},
+      {(void**)&ctx->UII, sbytes}, {(void**)&ctx->T, sbytes},   {(void**)&ctx->du, sbytes},
+      {(void**)&ctx->F, fbytes},   {(void**)&ctx->F0, fbytes},  {(void**)&ctx->FI, fbytes},
+      {(void**)&ctx->FII, fbytes}, {(void**)&ctx->Ft, fbytes},  {(void**)&ctx->dEz, fbytes},
+      {(void**)&ctx->FX, xbytes},  {(void**)&ctx->FY, xbytes},  {(void**)&ctx->EZX, ebytes},
+      {(void**)&ctx->EZY, ebytes}, {(void**)&ctx->EZV, (size_t)g.plane * sizeof(double)},
+      {(void**)&ctx->trouble, (size_t)g.Nx * g.Ny * sizeof(int)},
+      {(void**)&ctx->sc, sizeof(DevScal)},
+      {(void**)&ctx->partial, (size_t)ctx->nblk_cells * sizeof(double)},
+      {(void**)&ctx->icp, sizeof(ICPar)},
+  };
+  for (auto& a : al) {
+    e = hipMalloc(a.p, a.n);
+    if (e != hipSuccess) {
+      snprintf(ctx->err, sizeof ctx->err, "hipMalloc(%zu) failed: %s", a.n, hipGetErrorString(e));
+      mhd2d_destroy(ctx);
+      return -2;
+    }
+  }
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { mhd2d_destroy(ctx); return -2; }
+  // zero everything once (halo and padding included)
+  for (auto& a : al) (void)hipMemsetAsync(*a.p, 0, a.n, ctx->stream);
+  DevScal h{};
+  h.CFL = ctx->cfg.CFL;
+  h.pmin = 1e6;
+  h.active = 1;
+  (void)hipMemcpyAsync(ctx->sc, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream);
+  e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) { mhd2d_destroy(ctx); return -2; }
+  *out = ctx;
+  return 0;
+}
+
+int mhd2d_destroy(mhd2d_ctx* ctx) {
+  if (!ctx) return 0;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  void* ps[] = {ctx->S, ctx->U0, ctx->UI, ctx->UII, ctx->T, ctx->du, ctx->F, ctx->F0, ctx->FI, ctx->FII,
+                ctx->Ft, ctx->dEz, ctx->FX, ctx->FY, ctx->EZX, ctx->EZY, ctx->EZV, ctx->trouble, ctx->sc,
+                ctx->partial, ctx->icp};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  free(ctx);
+  return 0;
+}
+
+int mhd2d_sync(mhd2d_ctx* ctx) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// host uh(0:Nx+1,0:Ny+1,10,8) <-> device planes; Bx(0:Nx,1:Ny,3), By(1:Nx,0:Ny,3)
+int mhd2d_upload(mhd2d_ctx* ctx, const double* uh, const double* Bx, const double* By) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  const Geo& g = ctx->g;
+  const size_t W = g.Nx + 2, H = g.Ny + 2, P = g.P;
+  if (uh) {
+    for (int n = 0; n < 8; n++)
+      for (int b = 0; b < 10; b++) {
+        const int p = PL(b, n);
+        if (p < 0) continue;
+        const double* src = uh + W * H * (b + 10 * n);
+        HIPCHK(ctx, hipMemcpy2DAsync(ctx->S + p * g.plane + OFF, P * 8, src, W * 8, W * 8, H, hipMemcpyHostToDevice,
+                                     ctx->stream));
+      }
+  }
+  if (Bx)
+    for (int k = 0; k < 3; k++)
+      HIPCHK(ctx, hipMemcpy2DAsync(ctx->F + k * g.plane + cix(g, 0, 1), P * 8, Bx + (size_t)(g.Nx + 1) * g.Ny * k,
+                                   (g.Nx + 1) * 8, (g.Nx + 1) * 8, g.Ny, hipMemcpyHostToDevice, ctx->stream));
+  if (By)
+    for (int k = 0; k < 3; k++)
+      HIPCHK(ctx, hipMemcpy2DAsync(ctx->F + (3 + k) * g.plane + cix(g, 1, 0), P * 8, By + (size_t)g.Nx * (g.Ny + 1) * k,
+                                   g.Nx * 8, g.Nx * 8, g.Ny + 1, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->lh_done = false;
+  return 0;
+}
+
+static int download_planes(mhd2d_ctx* ctx, const double* dev, double* uh) {
+  const Geo& g = ctx->g;
+  const size_t W = g.Nx + 2, H = g.Ny + 2, P = g.P;
+  for (int n = 0; n < 8; n++)
+    for (int b = 0; b < 10; b++) {
+      double* dst = uh + W * H * (b + 10 * n);
+      const int p = PL(b, n);
+      if (p < 0) { memset(dst, 0, W * H * sizeof(double)); continue; }
+      HIPCHK(ctx, hipMemcpy2DAsync(dst, W * 8, dev + p * g.plane + OFF, P * 8, W * 8, H, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    }
+  return 0;
+}
+
+static int download_faces(mhd2d_ctx* ctx, const double* dev, double* Bx, double* By) {
+  const Geo& g = ctx->g;
+  const size_t P = g.P;
+  if (Bx)
+    for (int k = 0; k < 3; k++)
+      HIPCHK(ctx, hipMemcpy2DAsync(Bx + (size_t)(g.Nx + 1) * g.Ny * k, (g.Nx + 1) * 8, dev + k * g.plane + cix(g, 0, 1),
+                                   P * 8, (g.Nx + 1) * 8, g.Ny, hipMemcpyDeviceToHost, ctx->stream));
+  if (By)
+    for (int k = 0; k < 3; k++)
+      HIPCHK(ctx, hipMemcpy2DAsync(By + (size_t)g.Nx * (g.Ny + 1) * k, g.Nx * 8, dev + (3 + k) * g.plane + cix(g, 1, 0),
+                                   P * 8, g.Nx * 8, g.Ny + 1, hipMemcpyDeviceToHost, ctx->stream));
+  return 0;
+}
+
+int mhd2d_download(mhd2d_ctx* ctx, double* uh, double* Bx, double* By) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if (uh && (rc = download_planes(ctx, ctx->S, uh))) return rc;
+  if ((rc = download_faces(ctx, ctx->F, Bx, By))) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int mhd2d_debug_get(mhd2d_ctx* ctx, int32_t which, double* host) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  const Geo& g = ctx->g;
+  if (!host) return fail(ctx, -1, "debug_get: null host pointer");
+  switch (which) {
+    case MHD2D_DU: rc = download_planes(ctx, ctx->du, host); break;
+    case MHD2D_DEZ1: rc = download_faces(ctx, ctx->dEz, host, nullptr); break;
+    case MHD2D_DEZ2: rc = download_faces(ctx, ctx->dEz, nullptr, host); break;
+    case MHD2D_EZVERTEX:
+      HIPCHK(ctx, hipMemcpy2DAsync(host, (g.Nx + 1) * 8, ctx->EZV + cix(g, 0, 0), g.P * 8, (g.Nx + 1) * 8, g.Ny + 1,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+      break;
+    case MHD2D_TROUBLE: {
+      std::vector<int> tmp((size_t)g.Nx * g.Ny);
+      HIPCHK(ctx, hipMemcpyAsync(tmp.data(), ctx->trouble, tmp.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      for (size_t k = 0; k < tmp.size(); k++) host[k] = tmp[k];
+      return 0;
+    }
+    default: return fail(ctx, -1, "debug_get: unknown array %d", which);
+  }
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+static uint64_t splitmix64(uint64_t* x) {
+  uint64_t z = (*x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+int mhd2d_init_data(mhd2d_ctx* ctx, int32_t ic, const double* extra, int32_t nextra) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  ICPar h{};
+  h.ic = ic;
+  const double pi = 4 * atan(1.0);
+  if (ic == MHD2D_IC_OT_PERTURBED) {
+    h.par[0] = nextra > 0 ? extra[0] : 1e-2;
+    uint64_t st = nextra > 1 ? (uint64_t)extra[1] : 20261203ull;
+    for (int m = 0; m < 16; m++) {
+      h.kx[m] = 1 + (m % 4);
+      h.ky[m] = 1 + (m / 4);
+      h.phu[m] = 2 * pi * ((double)(splitmix64(&st) >> 11) * (1.0 / 9007199254740992.0));
+      h.phv[m] = 2 * pi * ((double)(splitmix64(&st) >> 11) * (1.0 / 9007199254740992.0));
+    }
+  } else if (ic == MHD2D_IC_ROTOR) {
+    const double def[8] = {0.1, 0.115, 1.0, 0.5, 2.4 / sqrt(4 * pi), 10.0, 0.5, 0.5};
+    for (int k = 0; k < 8; k++) h.par[k] = k < nextra ? extra[k] : def[k];
+  } else if (ic == MHD2D_IC_RIEMANN4) {
+    for (int k = 0; k < 26 && k < nextra; k++) h.par[k] = extra[k];
+  } else if (ic != MHD2D_IC_ORSZAG_TANG && ic != MHD2D_IC_VORTEX) {
+    return fail(ctx, -1, "init_data: unknown initial condition %d", ic);
+  }
+  const Geo& g = ctx->g;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->icp, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
+  const long long nc = (long long)(g.Nx + 2) * (g.Ny + 2);
+  const long long nf = (long long)(g.Nx + 1) * g.Ny + (long long)g.Nx * (g.Ny + 1);
+  hipLaunchKernelGGL(k_init_cells, dim3(nblk(nc)), dim3(BS), 0, ctx->stream, ctx->S, g, ctx->cfg.xa, ctx->cfg.ya,
+                     (const ICPar*)ctx->icp);
+  hipLaunchKernelGGL(k_init_faces, dim3(nblk(nf)), dim3(BS), 0, ctx->stream, ctx->F, g, ctx->cfg.xa, ctx->cfg.ya,
+                     (const ICPar*)ctx->icp);
+  LAUNCHCHK(ctx);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->lh_done = false;
+  return 0;
+}
+
+int mhd2d_set_bc(mhd2d_ctx* ctx) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  return launch_setbc(ctx, ctx->S, nullptr);
+}
+
+int mhd2d_lh(mhd2d_ctx* ctx) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if ((rc = launch_fluxes(ctx, ctx->S, false, nullptr))) return rc;
+  if ((rc = launch_cell(ctx, 0, ctx->S, nullptr, nullptr, ctx->du, nullptr, 0.0))) return rc;
+  ctx->lh_done = true;
+  return 0;
+}
+
+int mhd2d_lhez(mhd2d_ctx* ctx) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if (!ctx->lh_done) return fail(ctx, -5, "lhez: requires a preceding lh on the same state (LhEz.f90 reads Lh's traces)");
+  const Geo& g = ctx->g;
+  rc = dispatch_ft(ctx->cfg.flux_type, [&](auto FTc) -> int {
+    constexpr int FT = decltype(FTc)::value;
+    const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
+    hipLaunchKernelGGL(k_vertex<FT>, dim3(nblk(nv)), dim3(BS), 0, ctx->stream, ctx->S, ctx->EZV, g, nullptr);
+    LAUNCHCHK(ctx);
+    return 0;
+  });
+  if (rc) return rc;
+  return launch_ct(ctx, 0, nullptr, nullptr, nullptr, ctx->dEz, nullptr, 0.0);
+}
+
+// RK3.f90:157-168 (stage 1), :183-193 (stage 2), :206-212 (stage 3)
+int mhd2d_rk_stage(mhd2d_ctx* ctx, int32_t stage, double dt) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  const long long ns = (long long)NPL * ctx->g.plane, nf = 6LL * ctx->g.plane;
+  const size_t bs = ns * sizeof(double), bf = nf * sizeof(double);
+  hipStream_t st = ctx->stream;
+  auto ax = [&](double* out, const double* X, double a, const double* Y, double b, const double* Z, double d, long long n) {
+    hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(BS), 0, st, out, X, a, Y, b, Z, d, n);
+  };
+  if (stage == 1) {
+    ax(ctx->UI, nullptr, 0, ctx->S, 1, ctx->du, dt, ns);
+    ax(ctx->FI, nullptr, 0, ctx->F, 1, ctx->dEz, dt, nf);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->U0, ctx->S, bs, hipMemcpyDeviceToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->F0, ctx->F, bf, hipMemcpyDeviceToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->S, ctx->UI, bs, hipMemcpyDeviceToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->F, ctx->FI, bf, hipMemcpyDeviceToDevice, st));
+  } else if (stage == 2) {
+    ax(ctx->S, nullptr, 0, ctx->UI, 1, ctx->du, dt, ns);
+    ax(ctx->F, nullptr, 0, ctx->FI, 1, ctx->dEz, dt, nf);
+    ax(ctx->UII, ctx->U0, 3.0 / 4.0, ctx->S, 1.0 / 4.0, ctx->S, 0.0, ns);
+    ax(ctx->FII, ctx->F0, 3.0 / 4.0, ctx->F, 1.0 / 4.0, ctx->F, 0.0, nf);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->S, ctx->UII, bs, hipMemcpyDeviceToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->F, ctx->FII, bf, hipMemcpyDeviceToDevice, st));
+  } else if (stage == 3) {
+    ax(ctx->F, nullptr, 0, ctx->FII, 1, ctx->dEz, dt, nf);
+    ax(ctx->S, nullptr, 0, ctx->UII, 1, ctx->du, dt, ns);
+    ax(ctx->S, ctx->U0, 1.0 / 3.0, ctx->S, 2.0 / 3.0, ctx->S, 0.0, ns);
+    ax(ctx->F, ctx->F0, 1.0 / 3.0, ctx->F, 2.0 / 3.0, ctx->F, 0.0, nf);
+  } else {
+    return fail(ctx, -1, "rk_stage: stage must be 1, 2 or 3");
+  }
+  LAUNCHCHK(ctx);
+  ctx->lh_done = false;
+  return 0;
+}
+
+int mhd2d_tvb_limiter_p2(mhd2d_ctx* ctx) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if ((rc = launch_limit(ctx, ctx->S, ctx->T, ctx->F, true, ctx->cfg.pp != 0, false, nullptr))) return rc;
+  std::swap(ctx->S, ctx->T);
+  ctx->lh_done = false;
+  return launch_setbc(ctx, ctx->S, nullptr);   // TVB_Limiter_P2.f90:126
+}
+
+int mhd2d_pp_limiter(mhd2d_ctx* ctx) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if ((rc = launch_limit(ctx, ctx->S, ctx->T, ctx->F, false, true, false, nullptr))) return rc;
+  std::swap(ctx->S, ctx->T);
+  ctx->lh_done = false;
+  return 0;
+}
+
+int mhd2d_div_free_balsara(mhd2d_ctx* ctx) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if ((rc = launch_limit(ctx, ctx->S, ctx->T, ctx->F, false, false, true, nullptr))) return rc;
+  std::swap(ctx->S, ctx->T);
+  ctx->lh_done = false;
+  return 0;
+}
+
+int mhd2d_kxrcf_detector(mhd2d_ctx* ctx, int64_t* nflag) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, 1);
+  if ((rc = launch_kxrcf(ctx, ctx->S, ctx->sc))) return rc;
+  if (nflag) {
+    DevScal h;
+    HIPCHK(ctx, hipMemcpyAsync(&h, ctx->sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *nflag = h.nflag;
+  }
+  return 0;
+}
+
+static int launch_dt(mhd2d_ctx* ctx, int advance) {
+  const Geo& g = ctx->g;
+  hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, 0);
+  hipLaunchKernelGGL(k_dt, dim3(ctx->nblk_cells), dim3(BS), 0, ctx->stream, ctx->S, g, ctx->sc);
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, g.hx, g.hy, advance);
+  LAUNCHCHK(ctx);
+  return 0;
+}
+
+int mhd2d_calculate_dt(mhd2d_ctx* ctx, double* dt) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if ((rc = launch_dt(ctx, 0))) return rc;
+  DevScal h;
+  HIPCHK(ctx, hipMemcpyAsync(&h, ctx->sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (dt) *dt = h.dt;
+  return 0;
+}
+
+static int launch_diag(mhd2d_ctx* ctx) {
+  const Geo& g = ctx->g;
+  hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, 1);
+  hipLaunchKernelGGL(k_diag, dim3(ctx->nblk_cells), dim3(BS), 0, ctx->stream, ctx->S, g, ctx->sc, ctx->partial);
+  hipLaunchKernelGGL(k_div_final, dim3(1), dim3(BS), 0, ctx->stream, ctx->sc, (const double*)ctx->partial,
+                     ctx->nblk_cells, g.hx1, g.hy1);
+  LAUNCHCHK(ctx);
+  return 0;
+}
+
+int mhd2d_diagnostics(mhd2d_ctx* ctx, double* umax, double* totaldiv, double* pmin) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if ((rc = launch_diag(ctx))) return rc;
+  DevScal h;
+  HIPCHK(ctx, hipMemcpyAsync(&h, ctx->sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (umax) *umax = h.umax;
+  if (totaldiv) *totaldiv = h.totaldiv;
+  if (pmin) *pmin = h.pmin;
+  return 0;
+}
+
+int mhd2d_set_time(mhd2d_ctx* ctx, double t, int64_t count) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  DevScal h;
+  HIPCHK(ctx, hipMemcpyAsync(&h, ctx->sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  h.t = t;
+  h.count = count;
+  h.active = 1;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->sc, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int mhd2d_get_time(mhd2d_ctx* ctx, double* t, double* dt, int64_t* count) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  DevScal h;
+  HIPCHK(ctx, hipMemcpyAsync(&h, ctx->sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (t) *t = h.t;
+  if (dt) *dt = h.dt;
+  if (count) *count = h.count;
+  return 0;
+}
+
+// RK3.f90:82-95
+int mhd2d_rk3_prologue(mhd2d_ctx* ctx) {
+  int rc;
+  if ((rc = mhd2d_set_time(ctx, 0.0, 0))) return rc;
+  if ((rc = mhd2d_set_bc(ctx))) return rc;
+  if ((rc = mhd2d_tvb_limiter_p2(ctx))) return rc;
+  if ((rc = mhd2d_div_free_balsara(ctx))) return rc;
+  return launch_diag(ctx);
+}
+
+// one pass of the RK3.f90:135-228 loop body with t and dt on the device
+int mhd2d_step(mhd2d_ctx* ctx, double tend, double* t_out) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_set_tend, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, tend);
+  if ((rc = launch_dt(ctx, 1))) return rc;
+  DevScal* sc = ctx->sc;
+  const bool pp = ctx->cfg.pp != 0;
+  // stage 1: UI = S + dt L(S); S1 = limit(UI)  (U0 keeps S)
+  if ((rc = launch_fluxes(ctx, ctx->S, true, sc))) return rc;
+  if ((rc = launch_cell(ctx, 1, ctx->S, nullptr, nullptr, ctx->UI, sc, 0.0))) return rc;
+  if ((rc = launch_ct(ctx, 1, ctx->F, nullptr, nullptr, ctx->FI, sc, 0.0))) return rc;
+  if ((rc = launch_limit(ctx, ctx->UI, ctx->T, ctx->FI, true, pp, true, sc))) return rc;
+  if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;
+  std::swap(ctx->U0, ctx->S);   // U0 <- start-of-step state
+  std::swap(ctx->S, ctx->T);    // S  <- limited stage-1 state
+  std::swap(ctx->F0, ctx->F);   // F0 <- start-of-step faces; F is free
+  // stage 2: UII = 3/4 U0 + 1/4 (UI + dt L(S)); S2 = limit(UII)
+  if ((rc = launch_fluxes(ctx, ctx->S, true, sc))) return rc;
+  if ((rc = launch_cell(ctx, 2, ctx->S, ctx->UI, ctx->U0, ctx->UII, sc, 0.0))) return rc;
+  if ((rc = launch_ct(ctx, 2, nullptr, ctx->FI, ctx->F0, ctx->FII, sc, 0.0))) return rc;
+  if ((rc = launch_limit(ctx, ctx->UII, ctx->T, ctx->FII, true, pp, true, sc))) return rc;
+  if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;
+  std::swap(ctx->S, ctx->T);
+  // stage 3: T = 1/3 U0 + 2/3 (UII + dt L(S)); S_new = limit(T)
+  if ((rc = launch_fluxes(ctx, ctx->S, true, sc))) return rc;
+  if ((rc = launch_cell(ctx, 3, ctx->S, ctx->UII, ctx->U0, ctx->T, sc, 0.0))) return rc;
+  if ((rc = launch_ct(ctx, 3, nullptr, ctx->FII, ctx->F0, ctx->F, sc, 0.0))) return rc;
+  if ((rc = launch_limit(ctx, ctx->T, ctx->UI, ctx->F, true, pp, true, sc))) return rc;
+  if ((rc = launch_setbc(ctx, ctx->UI, sc))) return rc;
+  std::swap(ctx->S, ctx->UI);   // S <- new state; UI is free
+  ctx->lh_done = false;
+  if (t_out) {
+    DevScal h;
+    HIPCHK(ctx, hipMemcpyAsync(&h, ctx->sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *t_out = h.t;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+/*
+ * mhd2d.h -- C ABI of the MI355X (gfx950) implementation of one time step of
+ * the 2D ideal-MHD P2 globally divergence-free DG solver of
+ * GDF_Balsara_P2_0316 (L1ssandra/Freljord-CFD-arsenal).
+ *
+ * The reference exposes argument-less Fortran subroutines that share state
+ * through COMMON blocks (GDF_Balsara_P2_0316/com.txt:19-54).  Here that state
+ * lives in an opaque context on one GPU; each entry point below replaces one
+ * of those subroutines (cited per function) and keeps its meaning, including
+ * the order-dependent semantics of the reference's whole-array statements.
+ * Host arrays cross the boundary in the reference's own column-major layout:
+ *     uh(0:Nx+1, 0:Ny+1, 10, 8)   Bx(0:Nx, 1:Ny, 3)   By(1:Nx, 0:Ny, 3)
+ * so a Fortran caller passes its COMMON arrays unchanged (iso_c_binding).
+ *
+ * Every function returns 0 on success, a negative value on a HIP or usage
+ * error (message in mhd2d_last_error), and is stream-ordered on the
+ * context's stream; functions that return scalars synchronise.
+ * One host thread per context; contexts are independent.
+ */
+#ifndef MHD2D_H
+#define MHD2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mhd2d_ctx mhd2d_ctx;
+
+/* Riemann solver (com.txt:1 flux_type) */
+enum { MHD2D_LF = 1, MHD2D_HLL = 2, MHD2D_HLLC = 3, MHD2D_HLLD = 4 };
+/* which cells the TVB limiter touches */
+enum {
+  MHD2D_IND_NONE = 0,      /* no limiting                                   */
+  MHD2D_IND_ALL = 1,       /* every cell: TVB_Limiter_P2.f90 as shipped     */
+  MHD2D_IND_KXRCF = 2,     /* cells flagged by KXRCF_Detector (Ck)          */
+  MHD2D_IND_MLP = 3        /* learned indicator, KXRCF fallback in the band */
+};
+enum { MHD2D_FP64 = 0, MHD2D_MIXED = 1 };
+/* initial conditions (init_data.f90 + initN.txt) */
+enum {
+  MHD2D_IC_VORTEX = 1,           /* init1.txt                                */
+  MHD2D_IC_ORSZAG_TANG = 3,      /* init3.txt                                */
+  MHD2D_IC_ROTOR = 4,            /* init4.txt + Rotor_functions.f90          */
+  MHD2D_IC_OT_PERTURBED = 103,   /* Orszag-Tang + seeded 16-mode velocity    */
+  MHD2D_IC_RIEMANN4 = 104        /* four constant quadrants                  */
+};
+/* arrays readable with mhd2d_debug_get */
+enum { MHD2D_DU = 0, MHD2D_DEZ1 = 1, MHD2D_DEZ2 = 2, MHD2D_EZVERTEX = 3, MHD2D_TROUBLE = 4 };
+
+typedef struct mhd2d_config {
+  int32_t Nx, Ny;                 /* com.txt:1                                */
+  int32_t flux_type;              /* MHD2D_LF .. MHD2D_HLLD                   */
+  int32_t bcR, bcL, bcU, bcD;     /* 1 periodic, 2 outflow (set_bc.f90)       */
+  int32_t pp;                     /* pp_Limiter inside TVB (TVB_Limiter_P2.f90:123) */
+  int32_t indicator;              /* MHD2D_IND_*                              */
+  int32_t precision;              /* MHD2D_FP64 or MHD2D_MIXED                */
+  int32_t reserved;
+  double xa, xb, ya, yb;          /* domain (initN.txt)                       */
+  double M;                       /* TVB constant                             */
+  double CFL;                     /* calculate_dt.f90:8 (0.15 as shipped)     */
+  double Ck;                      /* KXRCF threshold                          */
+  double mlp_lo, mlp_hi;          /* MLP confidence band -> KXRCF fallback    */
+} mhd2d_config;
+
+/* context life cycle; allocates all device memory (no COMMON equivalent) */
+int mhd2d_create(const mhd2d_config *cfg, mhd2d_ctx **out);
+int mhd2d_destroy(mhd2d_ctx *ctx);
+const char *mhd2d_last_error(const mhd2d_ctx *ctx);
+int mhd2d_sync(mhd2d_ctx *ctx);
+
+/* host <-> device state in the reference layout (com.txt:22,48) */
+int mhd2d_upload(mhd2d_ctx *ctx, const double *uh, const double *Bx, const double *By);
+int mhd2d_download(mhd2d_ctx *ctx, double *uh, double *Bx, double *By);
+
+/* init_data.f90:21-121: L2 projection of an initial condition on the GPU.
+   extra: IC parameters (perturbed OT: amplitude, seed; rotor: r0, r1, spin,
+   p, Bx, rho_in, xc, yc; Riemann4: x0, y0, 4 x (rho,u,v,p,Bx,By)). */
+int mhd2d_init_data(mhd2d_ctx *ctx, int32_t ic, const double *extra, int32_t nextra);
+
+/* the reference's subroutines */
+int mhd2d_set_bc(mhd2d_ctx *ctx);                        /* set_bc.f90:5-27          */
+int mhd2d_lh(mhd2d_ctx *ctx);                            /* Lh.f90:1-305             */
+int mhd2d_lhez(mhd2d_ctx *ctx);                          /* LhEz.f90:1-96 (after lh) */
+int mhd2d_rk_stage(mhd2d_ctx *ctx, int32_t stage, double dt); /* RK3.f90:157-212     */
+int mhd2d_tvb_limiter_p2(mhd2d_ctx *ctx);                /* TVB_Limiter_P2.f90:1-191 */
+int mhd2d_pp_limiter(mhd2d_ctx *ctx);                    /* pp_Limiter.f90:1-83      */
+int mhd2d_kxrcf_detector(mhd2d_ctx *ctx, int64_t *nflag);/* KXRCF_Detector.f90:1-108 */
+int mhd2d_div_free_balsara(mhd2d_ctx *ctx);              /* div_free_Balsara.f90     */
+int mhd2d_calculate_dt(mhd2d_ctx *ctx, double *dt);      /* calculate_dt.f90:5-27    */
+int mhd2d_diagnostics(mhd2d_ctx *ctx, double *umax, double *totaldiv, double *pmin);
+                                   /* calculate_umax/_totaldiv/_pmin.f90          */
+int mhd2d_rk3_prologue(mhd2d_ctx *ctx);                  /* RK3.f90:8-18             */
+/* one pass of the RK3.f90:135-228 loop body (dt, clip to tend, 3 stages)
+   with t and dt kept on the device; t_out (may be NULL) synchronises. */
+int mhd2d_step(mhd2d_ctx *ctx, double tend, double *t_out);
+int mhd2d_get_time(mhd2d_ctx *ctx, double *t, double *dt, int64_t *count);
+int mhd2d_set_time(mhd2d_ctx *ctx, double t, int64_t count);
+/* copies of internal arrays in the reference layout (which = MHD2D_DU ..) */
+int mhd2d_debug_get(mhd2d_ctx *ctx, int32_t which, double *host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

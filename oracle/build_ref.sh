@@ -1,0 +1,96 @@
+#!/usr/bin/env bash
+# build_ref.sh -- compile the reference Fortran (GDF_Balsara_P2_0316) with
+# flang into oracle/_ref/, for use as the checker of the CPU restatement and
+# as the "reference" CPU baseline.  Runs only where /root/reference exists.
+#
+# The reference is never copied into this repository: each variant is built
+# in a throw-away directory under /tmp and only the linked binary lands in
+# oracle/_ref/ (git-ignored).  The reference fixes its configuration at
+# compile time, so each variant applies these configuration edits to the
+# scratch copy (the same edits the author makes by hand):
+#   P1 com.txt:1            Nx, Ny, flux_type; flash = 0 (no frame files)
+#   P2 init_data.f90:19     one object per initial condition (init3 = Orszag-
+#                           Tang, init1 = vortex, init4 = rotor), subroutine
+#                           renamed init_data_3 / _1 / _4
+#   P5 TVB_Limiter_P2.f90:123  'call pp_Limiter' enabled (variants *_pp)
+#   P7 HLL_Flux_2D.f90:8    EzRU..EzLD are read uninitialised at :176-177;
+#                           they are set to the corner fields v*B1 - u*B2
+#                           (calculate_Ez, as LF_Flux_2D.f90:18-21 does)
+# KXRCF_Detector.f90 does not compile as shipped (uG undeclared) and is not
+# built.  main.f90 is replaced by oracle/ref_driver.f90.
+set -euo pipefail
+
+REF=${REF:-/root/reference/GDF_Balsara_P2_0316}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+FC=${FC:-flang}
+JOBS=${JOBS:-8}
+
+if [ ! -d "$REF" ]; then
+  echo "build_ref.sh: $REF not present; reference build skipped" >&2
+  exit 0
+fi
+mkdir -p "$OUT"
+
+# name Nx Ny flux_type pp openmp
+VARIANTS=${VARIANTS:-"
+r16_f4 16 16 4 0 0
+r16_f2 16 16 2 0 0
+r16_f1 16 16 1 0 0
+r16_f3 16 16 3 0 0
+r16_f4_pp 16 16 4 1 0
+r12x8_f4 12 8 4 0 0
+r32_f4_pp 32 32 4 1 0
+r32_f2 32 32 2 0 0
+"}
+
+SRCS="get_GLP get_basis set_bc Lh LhEz eigenvalueMm HLL_Flux LF_Flux HLLC_Flux HLLD_Flux
+HLL_Flux_2D LF_Flux_2D HLLC_Flux_2D HLLD_Flux_2D HLL1 MHD_flux calculate_Ez TVB_Limiter_P2
+minmod minmodB eigenmatrix pp_Limiter pressure div_free_Balsara calculate_dt calculate_umax
+calculate_pmin calculate_totaldiv RK3 Rotor_functions"
+
+build_one() {
+  local name=$1 nx=$2 ny=$3 ft=$4 pp=$5 omp=$6
+  local w
+  w=$(mktemp -d /tmp/gdf_ref_XXXXXX)
+  cp "$REF"/*.f90 "$REF"/*.txt "$w"/
+  chmod -R u+w "$w"
+  # P1
+  sed -i "1s/Nx = [0-9]*, Ny = [0-9]*/Nx = $nx, Ny = $ny/; 1s/flux_type = [0-9]*/flux_type = $ft/; 1s/flash = [0-9]*/flash = 0/" "$w/com.txt"
+  # P2
+  for ic in 3 1 4; do
+    sed -e "s/include 'init6.txt'/include 'init$ic.txt'/" \
+        -e "s/subroutine init_data\b/subroutine init_data_$ic/" \
+        "$w/init_data.f90" > "$w/init_data_$ic.f90"
+  done
+  # P5
+  if [ "$pp" = 1 ]; then
+    sed -i "s/^\(\s*\)!call pp_Limiter/\1call pp_Limiter/" "$w/TVB_Limiter_P2.f90"
+  fi
+  # P7
+  sed -i "/^\s*ByLD = ULD1(7)/a\\
+    call calculate_Ez(EzRU,URU1(2)/URU1(1),URU1(3)/URU1(1),URU1(6),URU1(7))\\
+    call calculate_Ez(EzLU,ULU1(2)/ULU1(1),ULU1(3)/ULU1(1),ULU1(6),ULU1(7))\\
+    call calculate_Ez(EzRD,URD1(2)/URD1(1),URD1(3)/URD1(1),URD1(6),URD1(7))\\
+    call calculate_Ez(EzLD,ULD1(2)/ULD1(1),ULD1(3)/ULD1(1),ULD1(6),ULD1(7))" "$w/HLL_Flux_2D.f90"
+  cp "$HERE/ref_driver.f90" "$w/"
+  local flags="-O2 -fdefault-real-8 -fdefault-double-8"
+  local ldflags=""
+  if [ "$omp" = 1 ]; then flags="-O3 -fdefault-real-8 -fdefault-double-8 -fopenmp"; ldflags="-fopenmp"; fi
+  if [ $((nx * ny)) -gt 250000 ]; then flags="$flags -mcmodel=medium"; ldflags="$ldflags -mcmodel=medium -fuse-ld=bfd"; fi
+  (
+    cd "$w"
+    units="$SRCS init_data_3 init_data_1 init_data_4 ref_driver"
+    objs=""
+    for f in $units; do objs="$objs $f.o"; done
+    for f in $units; do echo "$FC $flags -c $f.f90"; done | xargs -P "$JOBS" -I{} sh -c "{}"
+    $FC $ldflags $objs -o "$OUT/$name"
+  )
+  rm -rf "$w"
+  echo "built $OUT/$name"
+}
+
+echo "$VARIANTS" | while read -r name nx ny ft pp omp; do
+  [ -z "$name" ] && continue
+  build_one "$name" "$nx" "$ny" "$ft" "$pp" "$omp"
+done
