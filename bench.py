@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Throughput of the 2D ideal-MHD P2 DG time step on one MI355X.
+
+Workload: seeded perturbed Orszag-Tang (16 velocity modes, amplitude 1e-2,
+seed 20261203) on [0, 2pi]^2 periodic at 2048 x 2048 cells, P2 (48 DOF per
+cell), HLLD fluxes and corner EMF, TVB limiter (M = 50) on every cell,
+positivity limiter on, Balsara divergence-free reconstruction, SSP-RK3,
+CFL 0.2.  A step is one pass of the reference's RK3 loop body: the dt
+reduction plus three stages.  Inputs are resident in HBM before timing.
+
+Prints ONE JSON line (rank 0).  ``value`` is DOF-updates/s
+(48 * N^2 * steps / seconds).
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "freljord-cfd-arsenal_amd")
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+DOF_PER_CELL = 48
+
+# algorithmic bytes per interior cell for one launch of each kernel class,
+# FP64 state of 52 planes (416 B/cell); see DESIGN.md "Kernels and rooflines"
+ALG_BYTES_FP64 = {
+    "xface": 416 + 45 * 8,          # state in, 40 fluxes + 5 face EMFs out
+    "yface": 416 + 45 * 8,
+    "vertex": 416 + 8,              # state in, one vertex EMF out
+    "cell": 416 + 640 + 416 + (0 + 832 + 832) / 3.0,   # state + fluxes in, stage value out, RK inputs
+    "ct": 2 * 40 + 8 + 48 + 48,     # face EMFs, vertex EMF, faces in and out
+    "limit": 416 + 48 + 416,        # stage value + faces in, limited state out
+    "setbc": 0.0,
+    "dt": 64,
+    "kxrcf": 416,
+    "diag": 416,
+}
+STEP_ALG_BYTES_PER_CELL = 2688     # 42 independent FP64 values x 8 state transfers per SSP-RK3 step
+
+
+def import_pkg():
+    name = "freljord_cfd_arsenal_amd"
+    spec = importlib.util.spec_from_file_location(name, os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(N=256, steps=2):
+    """the reference Fortran (OpenMP build in oracle/_ref) on a bounded
+    sample of the same workload: N x N cells, `steps` RK3 steps"""
+    from oracle import oracle as O
+    variant = f"r{N}_f4_pp_omp"
+    if not O.ref_available(variant):
+        return None
+    threads = 16    # Lh.f90:9 requests num_threads(16); the rest of the step is serial
+    o = O.Oracle(N, N, flux_type=O.HLLD, domain=(0.0, 2 * np.pi, 0.0, 2 * np.pi), M=50.0, CFL=0.2, pp=1,
+                 nthreads=1)
+    o.init_data(O.IC_OT_PERTURBED, [1e-2, 20261203])
+    env_backup = dict(os.environ)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    try:
+        r = O.run_ref(variant, "bench", o.uh, o.Bx, o.By, M=50.0, nsteps=steps, timeout=900)
+    finally:
+        os.environ.clear()
+        os.environ.update(env_backup)
+    secs = float(r["count"])      # the bench mode returns the timed seconds in this slot
+    return {"value": DOF_PER_CELL * N * N * steps / secs, "unit": "DOF-updates/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"reference Fortran (flang -O3 -fopenmp, OpenMP only in Lh) on the same workload at "
+                      f"{N}x{N}, {steps} RK3 steps (CFL 0.15 as shipped), {secs:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (no sharding, no collectives); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    pkg = import_pkg()
+    N = args.n
+    dom = (0.0, 2 * np.pi, 0.0, 2 * np.pi)
+    prec = pkg.FP64 if args.precision == "fp64" else pkg.MIXED
+    s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=pkg.IND_ALL,
+                   precision=prec)
+    s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
+    s.rk3_prologue()
+    for _ in range(args.warmup):
+        s.step()
+    s.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step()
+    s.sync()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    ms_per_step = 1e3 * secs / args.steps
+    value = DOF_PER_CELL * N * N * args.steps / secs
+    print(f"[bench] {args.steps} steps at {N}^2: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # live per-kernel timing (HIP events on the library's stream)
+    s.profile(True)
+    for _ in range(args.profile_steps):
+        s.step()
+    prof = s.profile_read()
+    s.profile(False)
+    dom_k = max(prof, key=lambda k: prof[k][0])
+    ms_tot, launches = prof[dom_k]
+    avg_ms = ms_tot / max(launches, 1)
+    bytes_launch = ALG_BYTES_FP64[dom_k] * N * N
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    step_gbs = STEP_ALG_BYTES_PER_CELL * N * N / (ms_per_step * 1e-3) / 1e9
+    d = s.diagnostics()
+    print("[bench] kernel ms per step: " + ", ".join(f"{k} {v[0] / args.profile_steps:.2f}" for k, v in prof.items()
+                                                     if v[1]), file=sys.stderr, flush=True)
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": "P2 MHD DOF-updates/s @2048^2: mixed+NN vs plain FP64, same GPU, at stated L2 err",
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == pkg.FP64 else "f64/f32",
+        "data": "synthetic",
+        "config": {"workload": f"perturbed Orszag-Tang {N}x{N}, P2 DG (48 DOF/cell), HLLD, TVB M=50 all cells, "
+                               f"positivity, SSP-RK3, CFL 0.2", "N": N, "form": args.precision,
+                   "indicator": "all cells (TVB_Limiter_P2 as shipped)", "totaldiv": d["totaldiv"],
+                   "pmin": d["pmin"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom_k,
+                     "kernel_avg_ms": avg_ms, "step_achieved": step_gbs,
+                     "step_frac": step_gbs / HBM_PEAK_GBS},
+        "cpu_baseline": cpu,
+        "kernels_ms_per_step": {k: v[0] / args.profile_steps for k, v in prof.items() if v[1]},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -33,7 +33,9 @@ EXPORTS = (
     "mhd2d_rk_stage", "mhd2d_tvb_limiter_p2", "mhd2d_pp_limiter", "mhd2d_kxrcf_detector",
     "mhd2d_div_free_balsara", "mhd2d_calculate_dt", "mhd2d_diagnostics", "mhd2d_rk3_prologue",
     "mhd2d_step", "mhd2d_get_time", "mhd2d_set_time", "mhd2d_debug_get",
+    "mhd2d_profile_enable", "mhd2d_profile_get",
 )
+KERNEL_CLASSES = ("xface", "yface", "vertex", "cell", "ct", "limit", "setbc", "dt", "kxrcf", "diag")
 
 
 class Config(C.Structure):
@@ -87,6 +89,8 @@ def load_library(path: str = LIB_PATH):
         "mhd2d_get_time": ([P, dp, dp, i64p], C.c_int),
         "mhd2d_set_time": ([P, C.c_double, C.c_int64], C.c_int),
         "mhd2d_debug_get": ([P, i32, dp], C.c_int),
+        "mhd2d_profile_enable": ([P, i32], C.c_int),
+        "mhd2d_profile_get": ([P, dp, i64p, i32], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -220,6 +224,17 @@ class Solver:
 
     def sync(self):
         self._chk(self.L.mhd2d_sync(self.h), "sync")
+
+    def profile(self, on=True):
+        self._chk(self.L.mhd2d_profile_enable(self.h, 1 if on else 0), "profile_enable")
+
+    def profile_read(self):
+        n = len(KERNEL_CLASSES)
+        ms = np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int64)
+        self._chk(self.L.mhd2d_profile_get(self.h, _dp(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64)), n),
+                  "profile_get")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
 
     def RK3(self, tend, max_steps=None):
         """RK3.f90:5-174 without frame output: prologue, then steps while t < tend."""

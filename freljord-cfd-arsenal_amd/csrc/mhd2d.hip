@@ -26,6 +26,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
+#include <utility>
 #include <vector>
 
 #include "../../include/mhd2d.h"
@@ -958,6 +960,10 @@ struct mhd2d_ctx {
   ICPar* icp;
   long long nblk_cells;
   bool lh_done;
+  // live kernel timing (mhd2d_profile_*): event pairs on the context stream
+  bool prof;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_ev;
+  std::vector<hipEvent_t> ev_pool;
   char err[512];
 };
 
@@ -987,6 +993,40 @@ int fail(mhd2d_ctx* ctx, int code, const char* fmt, ...) {
 
 inline unsigned nblk(long long n) { return (unsigned)((n + BS - 1) / BS); }
 
+// kernel classes for mhd2d_profile_get
+enum { KC_XFACE = 0, KC_YFACE, KC_VERTEX, KC_CELL, KC_CT, KC_LIMIT, KC_SETBC, KC_DT, KC_KXRCF, KC_DIAG, KC_N };
+
+hipEvent_t pool_event(mhd2d_ctx* ctx) {
+  if (!ctx->ev_pool.empty()) {
+    hipEvent_t e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// brackets one launch with events when profiling is on
+struct Timed {
+  mhd2d_ctx* ctx;
+  int kc;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(mhd2d_ctx* c, int k) : ctx(c), kc(k) {
+    if (ctx->prof) {
+      a = pool_event(ctx);
+      b = pool_event(ctx);
+      (void)hipEventRecord(a, ctx->stream);
+    }
+  }
+  ~Timed() {
+    if (ctx->prof && a && b) {
+      (void)hipEventRecord(b, ctx->stream);
+      ctx->prof_ev.push_back({kc, {a, b}});
+    }
+  }
+};
+
 template <typename F>
 int dispatch_ft(int ft, F&& f) {
   switch (ft) {
@@ -1003,9 +1043,16 @@ int launch_fluxes(mhd2d_ctx* ctx, const double* U, bool with_vertex, const DevSc
   return dispatch_ft(ctx->cfg.flux_type, [&](auto FTc) -> int {
     constexpr int FT = decltype(FTc)::value;
     const long long nxf = (long long)(g.Nx + 1) * g.Ny * 5, nyf = (long long)g.Nx * (g.Ny + 1) * 5;
-    hipLaunchKernelGGL(k_xface<FT>, dim3(nblk(nxf)), dim3(BS), 0, ctx->stream, U, ctx->FX, ctx->EZX, g, sc);
-    hipLaunchKernelGGL(k_yface<FT>, dim3(nblk(nyf)), dim3(BS), 0, ctx->stream, U, ctx->FY, ctx->EZY, g, sc);
+    {
+      Timed tm(ctx, KC_XFACE);
+      hipLaunchKernelGGL(k_xface<FT>, dim3(nblk(nxf)), dim3(BS), 0, ctx->stream, U, ctx->FX, ctx->EZX, g, sc);
+    }
+    {
+      Timed tm(ctx, KC_YFACE);
+      hipLaunchKernelGGL(k_yface<FT>, dim3(nblk(nyf)), dim3(BS), 0, ctx->stream, U, ctx->FY, ctx->EZY, g, sc);
+    }
     if (with_vertex) {
+      Timed tm(ctx, KC_VERTEX);
       const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
       hipLaunchKernelGGL(k_vertex<FT>, dim3(nblk(nv)), dim3(BS), 0, ctx->stream, U, ctx->EZV, g, sc);
     }
@@ -1018,6 +1065,7 @@ int launch_cell(mhd2d_ctx* ctx, int mode, const double* S, const double* A, cons
                 const DevScal* sc, double dt) {
   const Geo& g = ctx->g;
   const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
+  Timed tm(ctx, KC_CELL);
   switch (mode) {
     case 0: hipLaunchKernelGGL(k_cell<0>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, ctx->FX, ctx->FY, A, U0, out, g, sc, dt); break;
     case 1: hipLaunchKernelGGL(k_cell<1>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, ctx->FX, ctx->FY, A, U0, out, g, sc, dt); break;
@@ -1032,6 +1080,7 @@ int launch_ct(mhd2d_ctx* ctx, int mode, const double* F, const double* A, const 
               const DevScal* sc, double dt) {
   const Geo& g = ctx->g;
   const long long n = (long long)(g.Nx + 1) * g.Ny + (long long)g.Nx * (g.Ny + 1);
+  Timed tm(ctx, KC_CT);
   switch (mode) {
     case 0: hipLaunchKernelGGL(k_ct<0>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, ctx->EZX, ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt); break;
     case 1: hipLaunchKernelGGL(k_ct<1>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, ctx->EZX, ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt); break;
@@ -1045,6 +1094,7 @@ int launch_ct(mhd2d_ctx* ctx, int mode, const double* F, const double* A, const 
 int launch_setbc(mhd2d_ctx* ctx, double* U, const DevScal* sc) {
   const Geo& g = ctx->g;
   const long long ring = 2LL * (g.Nx + 2) + 2LL * g.Ny;
+  Timed tm(ctx, KC_SETBC);
   hipLaunchKernelGGL(k_setbc, dim3(nblk(ring * NPL)), dim3(BS), 0, ctx->stream, U, g, NPL, sc);
   LAUNCHCHK(ctx);
   return 0;
@@ -1052,6 +1102,7 @@ int launch_setbc(mhd2d_ctx* ctx, double* U, const DevScal* sc) {
 
 int launch_kxrcf(mhd2d_ctx* ctx, const double* U, DevScal* sc) {
   const Geo& g = ctx->g;
+  Timed tm(ctx, KC_KXRCF);
   hipLaunchKernelGGL(k_kxrcf, dim3(nblk((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream, U, ctx->trouble, g,
                      ctx->cfg.Ck, sc);
   LAUNCHCHK(ctx);
@@ -1074,6 +1125,7 @@ int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F,
   }
   const double M = ctx->cfg.M;
   const dim3 gr(nblk(n)), bl(BS);
+  Timed tm(ctx, KC_LIMIT);
 #define LIM(a, b, c) hipLaunchKernelGGL((k_limit<a, b, c>), gr, bl, 0, ctx->stream, in, out, F, trouble, g, M, sc)
   if (tvb && pp && div) LIM(true, true, true);
   else if (tvb && pp) LIM(true, true, false);
@@ -1114,6 +1166,8 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
   if (cfg->precision != MHD2D_FP64) return -4;   // only the plain form is built in this library version
   mhd2d_ctx* ctx = (mhd2d_ctx*)calloc(1, sizeof(mhd2d_ctx));
   if (!ctx) return -1;
+  new (&ctx->prof_ev) decltype(ctx->prof_ev)();
+  new (&ctx->ev_pool) decltype(ctx->ev_pool)();
   ctx->cfg = *cfg;
   if (ctx->cfg.CFL <= 0) ctx->cfg.CFL = 0.15;
   Geo& g = ctx->g;
@@ -1172,6 +1226,10 @@ int mhd2d_destroy(mhd2d_ctx* ctx) {
   if (!ctx) return 0;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& pe : ctx->prof_ev) { (void)hipEventDestroy(pe.second.first); (void)hipEventDestroy(pe.second.second); }
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  ctx->prof_ev.~vector();
+  ctx->ev_pool.~vector();
   void* ps[] = {ctx->S, ctx->U0, ctx->UI, ctx->UII, ctx->T, ctx->du, ctx->F, ctx->F0, ctx->FI, ctx->FII,
                 ctx->Ft, ctx->dEz, ctx->FX, ctx->FY, ctx->EZX, ctx->EZY, ctx->EZV, ctx->trouble, ctx->sc,
                 ctx->partial, ctx->icp};
@@ -1437,6 +1495,7 @@ int mhd2d_kxrcf_detector(mhd2d_ctx* ctx, int64_t* nflag) {
 
 static int launch_dt(mhd2d_ctx* ctx, int advance) {
   const Geo& g = ctx->g;
+  Timed tm(ctx, KC_DT);
   hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, 0);
   hipLaunchKernelGGL(k_dt, dim3(ctx->nblk_cells), dim3(BS), 0, ctx->stream, ctx->S, g, ctx->sc);
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, g.hx, g.hy, advance);
@@ -1457,6 +1516,7 @@ int mhd2d_calculate_dt(mhd2d_ctx* ctx, double* dt) {
 
 static int launch_diag(mhd2d_ctx* ctx) {
   const Geo& g = ctx->g;
+  Timed tm(ctx, KC_DIAG);
   hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, 1);
   hipLaunchKernelGGL(k_diag, dim3(ctx->nblk_cells), dim3(BS), 0, ctx->stream, ctx->S, g, ctx->sc, ctx->partial);
   hipLaunchKernelGGL(k_div_final, dim3(1), dim3(BS), 0, ctx->stream, ctx->sc, (const double*)ctx->partial,
@@ -1522,12 +1582,14 @@ int mhd2d_step(mhd2d_ctx* ctx, double tend, double* t_out) {
   if ((rc = launch_dt(ctx, 1))) return rc;
   DevScal* sc = ctx->sc;
   const bool pp = ctx->cfg.pp != 0;
-  // stage 1: UI = S + dt L(S); S1 = limit(UI)  (U0 keeps S)
+  // stage 1 (RK3.f90:151-174): set_bc; UI = S + dt L(S); S1 = limit(UI);
+  // U0 keeps S with its fresh halos
+  if ((rc = launch_setbc(ctx, ctx->S, sc))) return rc;
   if ((rc = launch_fluxes(ctx, ctx->S, true, sc))) return rc;
   if ((rc = launch_cell(ctx, 1, ctx->S, nullptr, nullptr, ctx->UI, sc, 0.0))) return rc;
   if ((rc = launch_ct(ctx, 1, ctx->F, nullptr, nullptr, ctx->FI, sc, 0.0))) return rc;
   if ((rc = launch_limit(ctx, ctx->UI, ctx->T, ctx->FI, true, pp, true, sc))) return rc;
-  if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;
+  if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;          // RK3.f90:174
   std::swap(ctx->U0, ctx->S);   // U0 <- start-of-step state
   std::swap(ctx->S, ctx->T);    // S  <- limited stage-1 state
   std::swap(ctx->F0, ctx->F);   // F0 <- start-of-step faces; F is free
@@ -1536,21 +1598,45 @@ int mhd2d_step(mhd2d_ctx* ctx, double tend, double* t_out) {
   if ((rc = launch_cell(ctx, 2, ctx->S, ctx->UI, ctx->U0, ctx->UII, sc, 0.0))) return rc;
   if ((rc = launch_ct(ctx, 2, nullptr, ctx->FI, ctx->F0, ctx->FII, sc, 0.0))) return rc;
   if ((rc = launch_limit(ctx, ctx->UII, ctx->T, ctx->FII, true, pp, true, sc))) return rc;
-  if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;
+  if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;          // RK3.f90:200 (stage 3 set_bc)
   std::swap(ctx->S, ctx->T);
   // stage 3: T = 1/3 U0 + 2/3 (UII + dt L(S)); S_new = limit(T)
   if ((rc = launch_fluxes(ctx, ctx->S, true, sc))) return rc;
   if ((rc = launch_cell(ctx, 3, ctx->S, ctx->UII, ctx->U0, ctx->T, sc, 0.0))) return rc;
   if ((rc = launch_ct(ctx, 3, nullptr, ctx->FII, ctx->F0, ctx->F, sc, 0.0))) return rc;
   if ((rc = launch_limit(ctx, ctx->T, ctx->UI, ctx->F, true, pp, true, sc))) return rc;
-  if ((rc = launch_setbc(ctx, ctx->UI, sc))) return rc;
-  std::swap(ctx->S, ctx->UI);   // S <- new state; UI is free
+  std::swap(ctx->S, ctx->UI);   // S <- new state (halos refreshed by the next step's set_bc); UI is free
   ctx->lh_done = false;
   if (t_out) {
     DevScal h;
     HIPCHK(ctx, hipMemcpyAsync(&h, ctx->sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     *t_out = h.t;
+  }
+  return 0;
+}
+
+// live kernel timing with HIP events on the context stream (no counterpart
+// in the reference): enable, run, then read per-class totals.
+int mhd2d_profile_enable(mhd2d_ctx* ctx, int32_t on) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto& pe : ctx->prof_ev) { ctx->ev_pool.push_back(pe.second.first); ctx->ev_pool.push_back(pe.second.second); }
+  ctx->prof_ev.clear();
+  ctx->prof = on != 0;
+  return 0;
+}
+
+int mhd2d_profile_get(mhd2d_ctx* ctx, double* ms, int64_t* launches, int32_t nclass) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < nclass; k++) { ms[k] = 0.0; launches[k] = 0; }
+  for (auto& pe : ctx->prof_ev) {
+    float e = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&e, pe.second.first, pe.second.second));
+    if (pe.first < nclass) { ms[pe.first] += e; launches[pe.first] += 1; }
   }
   return 0;
 }

