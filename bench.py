@@ -85,7 +85,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--error", type=int, default=1, help="report mixed-vs-FP64 L2 errors (downloads both states)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
     args = ap.parse_args()
@@ -97,40 +98,61 @@ def main():
     pkg = import_pkg()
     N = args.n
     dom = (0.0, 2 * np.pi, 0.0, 2 * np.pi)
-    prec = pkg.FP64 if args.precision == "fp64" else pkg.MIXED
-    s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=pkg.IND_ALL,
-                   precision=prec)
-    s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
-    s.rk3_prologue()
-    for _ in range(args.warmup):
-        s.step()
-    s.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s.step()
-    s.sync()
-    t1 = time.perf_counter()
-    secs = t1 - t0
-    ms_per_step = 1e3 * secs / args.steps
-    value = DOF_PER_CELL * N * N * args.steps / secs
-    print(f"[bench] {args.steps} steps at {N}^2: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
 
-    # live per-kernel timing (HIP events on the library's stream)
-    s.profile(True)
-    for _ in range(args.profile_steps):
-        s.step()
-    prof = s.profile_read()
-    s.profile(False)
+    def run(prec, steps, warmup, profile_steps):
+        s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=pkg.IND_ALL,
+                       precision=prec)
+        s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
+        s.rk3_prologue()
+        for _ in range(warmup):
+            s.step()
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step()
+        s.sync()
+        secs = time.perf_counter() - t0
+        state = s.download() if args.error else None
+        diag = s.diagnostics()
+        tnow = s.get_time()[0]
+        prof = None
+        if profile_steps:
+            s.profile(True)
+            for _ in range(profile_steps):
+                s.step()
+            prof = s.profile_read()
+            s.profile(False)
+        s.close()
+        return secs, state, diag, tnow, prof
+
+    forms = {"fp64": pkg.FP64, "mixed": pkg.MIXED}
+    results = {}
+    for name in (["fp64", "mixed"] if args.precision == "mixed" else ["fp64"]):
+        secs, state, diag, tnow, prof = run(forms[name], args.steps, args.warmup, args.profile_steps)
+        results[name] = dict(secs=secs, state=state, diag=diag, t=tnow, prof=prof,
+                             ms=1e3 * secs / args.steps, value=DOF_PER_CELL * N * N * args.steps / secs)
+        print(f"[bench] {name}: {args.steps} steps at {N}^2: {results[name]['ms']:.3f} ms/step, t={tnow:.6f}",
+              file=sys.stderr, flush=True)
+        print(f"[bench] {name} kernel ms per step: " +
+              ", ".join(f"{k} {v[0] / args.profile_steps:.2f}" for k, v in prof.items() if v[1]), file=sys.stderr,
+              flush=True)
+    head = results[args.precision]
+    ms_per_step, value, prof, d = head["ms"], head["value"], head["prof"], head["diag"]
+    errors = None
+    if args.precision == "mixed" and args.error:
+        a, b = results["mixed"]["state"][0][1:-1, 1:-1], results["fp64"]["state"][0][1:-1, 1:-1]
+        names = ("rho", "rhou", "rhov", "rhow", "E", "Bx", "By", "Bz")
+        errors = {}
+        for n, nm in enumerate(names):
+            den = np.linalg.norm(b[:, :, :, n])
+            if den > 0:
+                errors[nm] = float(np.linalg.norm(a[:, :, :, n] - b[:, :, :, n]) / den)
     dom_k = max(prof, key=lambda k: prof[k][0])
     ms_tot, launches = prof[dom_k]
     avg_ms = ms_tot / max(launches, 1)
     bytes_launch = ALG_BYTES_FP64[dom_k] * N * N
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     step_gbs = STEP_ALG_BYTES_PER_CELL * N * N / (ms_per_step * 1e-3) / 1e9
-    d = s.diagnostics()
-    print("[bench] kernel ms per step: " + ", ".join(f"{k} {v[0] / args.profile_steps:.2f}" for k, v in prof.items()
-                                                     if v[1]), file=sys.stderr, flush=True)
-
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": "P2 MHD DOF-updates/s @2048^2: mixed+NN vs plain FP64, same GPU, at stated L2 err",
@@ -143,18 +165,23 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == pkg.FP64 else "f64/f32",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"perturbed Orszag-Tang {N}x{N}, P2 DG (48 DOF/cell), HLLD, TVB M=50 all cells, "
                                f"positivity, SSP-RK3, CFL 0.2", "N": N, "form": args.precision,
                    "indicator": "all cells (TVB_Limiter_P2 as shipped)", "totaldiv": d["totaldiv"],
-                   "pmin": d["pmin"]},
+                   "pmin": d["pmin"],
+                   "fp64_value": results["fp64"]["value"], "fp64_ms_per_step": results["fp64"]["ms"],
+                   "speedup_vs_fp64": value / results["fp64"]["value"],
+                   "rel_l2_vs_fp64": errors},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom_k,
                      "kernel_avg_ms": avg_ms, "step_achieved": step_gbs,
                      "step_frac": step_gbs / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v[0] / args.profile_steps for k, v in prof.items() if v[1]},
+        "kernels_ms_per_step_fp64": {k: v[0] / args.profile_steps for k, v in results["fp64"]["prof"].items()
+                                     if v[1]},
     }
     print(json.dumps(out))
 

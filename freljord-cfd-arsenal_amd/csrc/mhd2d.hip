@@ -79,10 +79,10 @@ __device__ __forceinline__ double sel5(int q, double a0, double a1, double a2, d
   return r;
 }
 
-// trace of all 8 variables of one cell on one side at node q
-template <int SIDE>
+// trace of all 8 variables of one cell on one side at node q, in type T
+template <int SIDE, typename T>
 __device__ __forceinline__ void trace_at(const double* __restrict__ U, long long plane, long long c, int q,
-                                         double out[8]) {
+                                         T out[8]) {
   constexpr double L0 = lam(0), L1 = lam(1), L2 = lam(2), L3 = lam(3), L4 = lam(4);
   const double s = sel5(q, L0, L1, L2, L3, L4);
   const double p2 = sel5(q, P2(L0), P2(L1), P2(L2), P2(L3), P2(L4));
@@ -90,11 +90,11 @@ __device__ __forceinline__ void trace_at(const double* __restrict__ U, long long
   const double t23 = sel5(q, T23(L0), T23(L1), T23(L2), T23(L3), T23(L4));
 #pragma unroll
   for (int n = 0; n < 8; n++) {
-    double acc = 0.0;
+    T acc = T(0);
 #pragma unroll
     for (int b = 0; b < 10; b++) {
       const int p = PL(b, n);
-      if (p >= 0) acc += U[p * plane + c] * phiTr(SIDE, b, s, p2, p3, t23);
+      if (p >= 0) acc += T(U[p * plane + c]) * T(phiTr(SIDE, b, s, p2, p3, t23));
     }
     out[n] = acc;
   }
@@ -104,58 +104,58 @@ __device__ __forceinline__ void trace_at(const double* __restrict__ U, long long
 // x faces: Lh.f90:72-136 (traces, fluxes), :206-234 (Riemann),
 // LhEz.f90:6-16 (face EMF).  Face i (0..Nx) of row j (1..Ny) lies between
 // cells i and i+1.
-template <int FT>
-__global__ __launch_bounds__(BS) void k_xface(const double* __restrict__ U, double* __restrict__ FX,
-                                              double* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
+template <int FT, typename T>
+__global__ __launch_bounds__(BS) void k_xface(const double* __restrict__ U, T* __restrict__ FX,
+                                              T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
   const long long t = xcd_block() * BS + threadIdx.x;
   const long long nf = (long long)(g.Nx + 1) * g.Ny;
   if (t >= nf * 5) return;
   const int q = (int)(t % 5);
   const long long f = t / 5;
   const int i = (int)(f % (g.Nx + 1)), j = 1 + (int)(f / (g.Nx + 1));
-  double UL[8], UR[8], FL[8], FR[8], F[8];
-  trace_at<0>(U, g.plane, cix(g, i, j), q, UL);       // right trace of cell i
-  trace_at<1>(U, g.plane, cix(g, i + 1, j), q, UR);   // left trace of cell i+1
-  lh_flux<double, 1>(UL, FL);
-  lh_flux<double, 1>(UR, FR);
-  double a1, b1, a2, b2;
-  eigenvalueMm<double, 1>(UL, a1, b1);
-  eigenvalueMm<double, 1>(UR, a2, b2);
-  const double SR = tmax(a1, a2), SL = tmin(b1, b2);
-  riemann<double, 1, FT>(SR, SL, UL, UR, FL, FR, F);
+  T UL[8], UR[8], FL[8], FR[8], F[8];
+  trace_at<0, T>(U, g.plane, cix(g, i, j), q, UL);       // right trace of cell i
+  trace_at<1, T>(U, g.plane, cix(g, i + 1, j), q, UR);   // left trace of cell i+1
+  lh_flux<T, 1>(UL, FL);
+  lh_flux<T, 1>(UR, FR);
+  T a1, b1, a2, b2;
+  eigenvalueMm<T, 1>(UL, a1, b1);
+  eigenvalueMm<T, 1>(UR, a2, b2);
+  const T SR = tmax(a1, a2), SL = tmin(b1, b2);
+  riemann<T, 1, FT>(SR, SL, UL, UR, FL, FR, F);
   const long long o = cix(g, i, j);
 #pragma unroll
   for (int n = 0; n < 8; n++) FX[(q * 8 + n) * g.plane + o] = F[n];
-  const double S = tmax(fabs(UL[1] / UL[0]), fabs(UR[1] / UR[0]));
-  EZ[q * g.plane + o] = -0.5 * (FL[6] + FR[6] - S * (UR[6] - UL[6]));
+  const T S = tmax(fabs(UL[1] / UL[0]), fabs(UR[1] / UR[0]));
+  EZ[q * g.plane + o] = T(-0.5) * (FL[6] + FR[6] - S * (UR[6] - UL[6]));
 }
 
 // y faces: Lh.f90:139-203, :237-264; LhEz.f90:18-28.  Face j (0..Ny) of
 // column i (1..Nx) lies between cells j and j+1.
-template <int FT>
-__global__ __launch_bounds__(BS) void k_yface(const double* __restrict__ U, double* __restrict__ FY,
-                                              double* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
+template <int FT, typename T>
+__global__ __launch_bounds__(BS) void k_yface(const double* __restrict__ U, T* __restrict__ FY,
+                                              T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
   const long long t = xcd_block() * BS + threadIdx.x;
   const long long nf = (long long)g.Nx * (g.Ny + 1);
   if (t >= nf * 5) return;
   const int q = (int)(t % 5);
   const long long f = t / 5;
   const int i = 1 + (int)(f % g.Nx), j = (int)(f / g.Nx);
-  double UD[8], UU[8], FD[8], FU[8], F[8];
-  trace_at<2>(U, g.plane, cix(g, i, j), q, UD);       // upper trace of cell j (reference UU)
-  trace_at<3>(U, g.plane, cix(g, i, j + 1), q, UU);   // lower trace of cell j+1 (reference UD)
-  lh_flux<double, 2, false>(UD, FD);
-  lh_flux<double, 2, true>(UU, FU);
-  double a1, b1, a2, b2;
-  eigenvalueMm<double, 2>(UD, a1, b1);
-  eigenvalueMm<double, 2>(UU, a2, b2);
-  const double SR = tmax(a1, a2), SL = tmin(b1, b2);
-  riemann<double, 2, FT>(SR, SL, UD, UU, FD, FU, F);
+  T UD[8], UU[8], FD[8], FU[8], F[8];
+  trace_at<2, T>(U, g.plane, cix(g, i, j), q, UD);       // upper trace of cell j (reference UU)
+  trace_at<3, T>(U, g.plane, cix(g, i, j + 1), q, UU);   // lower trace of cell j+1 (reference UD)
+  lh_flux<T, 2, false>(UD, FD);
+  lh_flux<T, 2, true>(UU, FU);
+  T a1, b1, a2, b2;
+  eigenvalueMm<T, 2>(UD, a1, b1);
+  eigenvalueMm<T, 2>(UU, a2, b2);
+  const T SR = tmax(a1, a2), SL = tmin(b1, b2);
+  riemann<T, 2, FT>(SR, SL, UD, UU, FD, FU, F);
   const long long o = cix(g, i, j);
 #pragma unroll
   for (int n = 0; n < 8; n++) FY[(q * 8 + n) * g.plane + o] = F[n];
-  const double S = tmax(fabs(UD[2] / UD[0]), fabs(UU[2] / UU[0]));
-  EZ[q * g.plane + o] = 0.5 * (FD[5] + FU[5] - S * (UU[5] - UD[5]));
+  const T S = tmax(fabs(UD[2] / UD[0]), fabs(UU[2] / UU[0]));
+  EZ[q * g.plane + o] = T(0.5) * (FD[5] + FU[5] - S * (UU[5] - UD[5]));
 }
 
 // vertex (corner) EMF: LhEz.f90:33-69.  Vertex (i, j) is the upper-right
@@ -199,9 +199,19 @@ __global__ __launch_bounds__(BS) void k_vertex(const double* __restrict__ U, dou
 // The launch covers the halo ring too: there du = 0 (Lh.f90:5), so the
 // combination carries the start-of-step halos exactly as the reference's
 // whole-array statements do (the stale halos TVB later reads).
-template <int MODE>
-__global__ __launch_bounds__(BS) void k_cell(const double* __restrict__ S, const double* __restrict__ FX,
-                                             const double* __restrict__ FY, const double* __restrict__ A,
+//
+// T is the quadrature type (double: plain form; float: mixed form).  The
+// cell-average (mode 1) residual is accumulated in double in both forms.
+// The volume integral is sum-factorised row by row in eta: per row the
+// modes collapse to 1D coefficients in xi, and the modes are re-read for
+// every row (L1 hits) instead of being held, which keeps the accumulators
+// (52 per cell) and the row state inside the register file.
+template <typename T>
+__device__ __forceinline__ T ld(const double* __restrict__ p) { return (T)(*p); }
+
+template <int MODE, typename T>
+__global__ __launch_bounds__(BS) void k_cell(const double* __restrict__ S, const T* __restrict__ FX,
+                                             const T* __restrict__ FY, const double* __restrict__ A,
                                              const double* __restrict__ U0, double* __restrict__ out, Geo g,
                                              const DevScal* __restrict__ sc, double dt_arg) {
   const long long t = xcd_block() * BS + threadIdx.x;
@@ -212,32 +222,48 @@ __global__ __launch_bounds__(BS) void k_cell(const double* __restrict__ S, const
   const long long pl = g.plane;
   const double dt = sc ? sc->dt : dt_arg;
   const bool interior = i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny;
-  double acc[NPL];
+  double acc0[8];       // mode-1 (cell average) residuals
+  T acc[NPL];           // higher modes (slots PL(0,n) unused)
 #pragma unroll
-  for (int p = 0; p < NPL; p++) acc[p] = 0.0;
+  for (int n = 0; n < 8; n++) acc0[n] = 0.0;
+#pragma unroll
+  for (int p = 0; p < NPL; p++) acc[p] = T(0);
   if (interior) {
-    double m[NPL];
+    const T ihx1 = T(1.0 / g.hx1), ihy1 = T(1.0 / g.hy1);
 #pragma unroll
-    for (int p = 0; p < NPL; p++) m[p] = S[p * pl + c];
-    const double ihx1 = 1.0 / g.hx1, ihy1 = 1.0 / g.hy1;
-    // volume integral over the 5 x 5 Gauss points
+    for (int j1 = 0; j1 < 5; j1++) {
+      const double y = lam(j1);
+      const T ty = T(y), tP2y = T(P2(y)), tP3y = T(P3(y));
+      long long cj = c;
+      asm volatile("" : "+v"(cj));   // re-read the modes for every row
+      // U(x, y_j) = c0 + c1 x + c2 P2(x) + c3 P3(x) per variable
+      T c0[8], c1[8], c2[8], c3b = T(0);
 #pragma unroll
-    for (int i1 = 0; i1 < 5; i1++) {
-#pragma unroll
-      for (int j1 = 0; j1 < 5; j1++) {
-        const double x = lam(i1), y = lam(j1);
-        double Uq[8], Fx[8], Fy[8];
-#pragma unroll
-        for (int n = 0; n < 8; n++) {
-          double a = 0.0;
-#pragma unroll
-          for (int b = 0; b < 10; b++) {
-            const int p = PL(b, n);
-            if (p >= 0) a += m[p] * phi(b, x, y);
-          }
-          Uq[n] = a;
+      for (int n = 0; n < 8; n++) {
+        const T m0 = ld<T>(S + PL(0, n) * pl + cj), m1 = ld<T>(S + PL(1, n) * pl + cj);
+        const T m2 = ld<T>(S + PL(2, n) * pl + cj), m3 = ld<T>(S + PL(3, n) * pl + cj);
+        const T m4 = ld<T>(S + PL(4, n) * pl + cj), m5 = ld<T>(S + PL(5, n) * pl + cj);
+        c0[n] = m0 + m2 * ty + m5 * tP2y;
+        c1[n] = m1 + m4 * ty;
+        c2[n] = m3;
+        if (n == 5) {   // Bx: phi7 = P3(xi), phi9 = xi P2(eta)
+          c1[n] += ld<T>(S + PL(8, 5) * pl + cj) * tP2y;
+          c3b = ld<T>(S + PL(6, 5) * pl + cj);
         }
-        lh_flux_xy<double>(Uq, Fx, Fy);
+        if (n == 6) {   // By: phi8 = P2(xi) eta, phi10 = P3(eta)
+          c2[n] += ld<T>(S + PL(7, 6) * pl + cj) * ty;
+          c0[n] += ld<T>(S + PL(9, 6) * pl + cj) * tP3y;
+        }
+      }
+#pragma unroll
+      for (int i1 = 0; i1 < 5; i1++) {
+        const double x = lam(i1);
+        const T tx = T(x), tP2x = T(P2(x)), tP3x = T(P3(x));
+        T Uq[8], Fx[8], Fy[8];
+#pragma unroll
+        for (int n = 0; n < 8; n++) Uq[n] = c0[n] + c1[n] * tx + c2[n] * tP2x;
+        Uq[5] += c3b * tP3x;
+        lh_flux_xy<T>(Uq, Fx, Fy);
 #pragma unroll
         for (int n = 0; n < 8; n++) { Fx[n] *= ihx1; Fy[n] *= ihy1; }
         const double ww = 0.25 * wgt(i1) * wgt(j1);
@@ -248,14 +274,15 @@ __global__ __launch_bounds__(BS) void k_cell(const double* __restrict__ S, const
             const int p = PL(b, n);
             if (p < 0) continue;
             const double cx = ww * dphix(b, x, y), cy = ww * dphiy(b, x, y);
-            if (cx != 0.0) acc[p] += cx * Fx[n];
-            if (cy != 0.0) acc[p] += cy * Fy[n];
+            if (cx != 0.0) acc[p] += T(cx) * Fx[n];
+            if (cy != 0.0) acc[p] += T(cy) * Fy[n];
           }
         }
       }
     }
     // surface integrals
-    const double sx = 0.5 / g.hx, sy = 0.5 / g.hy;
+    const T sx = T(0.5 / g.hx), sy = T(0.5 / g.hy);
+    const double sxd = 0.5 / g.hx, syd = 0.5 / g.hy;
     const long long cL = cix(g, i - 1, j), cD = cix(g, i, j - 1);
 #pragma unroll
     for (int q = 0; q < 5; q++) {
@@ -263,34 +290,32 @@ __global__ __launch_bounds__(BS) void k_cell(const double* __restrict__ S, const
       const double wq = wgt(q);
 #pragma unroll
       for (int n = 0; n < 8; n++) {
-        const double fr = FX[(q * 8 + n) * pl + c], fl = FX[(q * 8 + n) * pl + cL];
-        const double fu = FY[(q * 8 + n) * pl + c], fd = FY[(q * 8 + n) * pl + cD];
+        const T fr = FX[(q * 8 + n) * pl + c], fl = FX[(q * 8 + n) * pl + cL];
+        const T fu = FY[(q * 8 + n) * pl + c], fd = FY[(q * 8 + n) * pl + cD];
+        acc0[n] -= sxd * wq * ((double)fr - (double)fl);
+        acc0[n] -= syd * wq * ((double)fu - (double)fd);
 #pragma unroll
-        for (int b = 0; b < 10; b++) {
+        for (int b = 1; b < 10; b++) {
           const int p = PL(b, n);
           if (p < 0) continue;
-          acc[p] -= sx * wq * (fr * phiTr(0, b, s, p2, p3, t23) - fl * phiTr(1, b, s, p2, p3, t23));
-          acc[p] -= sy * wq * (fu * phiTr(2, b, s, p2, p3, t23) - fd * phiTr(3, b, s, p2, p3, t23));
+          acc[p] -= sx * T(wq) * (fr * T(phiTr(0, b, s, p2, p3, t23)) - fl * T(phiTr(1, b, s, p2, p3, t23)));
+          acc[p] -= sy * T(wq) * (fu * T(phiTr(2, b, s, p2, p3, t23)) - fd * T(phiTr(3, b, s, p2, p3, t23)));
         }
       }
     }
-#pragma unroll
-    for (int n = 0; n < 8; n++)
-#pragma unroll
-      for (int b = 0; b < 10; b++) {
-        const int p = PL(b, n);
-        if (p >= 0) acc[p] *= (1.0 / mmass(b));
-      }
   }
-  // epilogue
+  // epilogue: mass inverse + RK combination, in double
 #pragma unroll
   for (int p = 0; p < NPL; p++) {
     const long long o = p * pl + c;
+    const bool avg = p < 48 && (p % 6) == 0;
+    int b = p < 48 ? p % 6 : (p == 48 ? 6 : p == 49 ? 8 : p == 50 ? 7 : 9);
+    const double du = avg ? acc0[p / 6] : (double)acc[p] * (1.0 / mmass(b));
     double r;
-    if (MODE == 0) r = acc[p];
-    else if (MODE == 1) r = S[o] + dt * acc[p];
-    else if (MODE == 2) r = (3.0 / 4.0) * U0[o] + (1.0 / 4.0) * (A[o] + dt * acc[p]);
-    else r = (1.0 / 3.0) * U0[o] + (2.0 / 3.0) * (A[o] + dt * acc[p]);
+    if (MODE == 0) r = du;
+    else if (MODE == 1) r = S[o] + dt * du;
+    else if (MODE == 2) r = (3.0 / 4.0) * U0[o] + (1.0 / 4.0) * (A[o] + dt * du);
+    else r = (1.0 / 3.0) * U0[o] + (2.0 / 3.0) * (A[o] + dt * du);
     out[o] = r;
   }
 }
@@ -299,8 +324,8 @@ __global__ __launch_bounds__(BS) void k_cell(const double* __restrict__ S, const
 // RK3.f90:159-212 for MODE 1..3 as in k_cell).  Face planes: 0..2 Bx modes
 // on x faces (i = 0..Nx, j = 1..Ny), 3..5 By modes on y faces (i = 1..Nx,
 // j = 0..Ny).
-template <int MODE>
-__global__ __launch_bounds__(BS) void k_ct(const double* __restrict__ EZX, const double* __restrict__ EZY,
+template <int MODE, typename T>
+__global__ __launch_bounds__(BS) void k_ct(const T* __restrict__ EZX, const T* __restrict__ EZY,
                                            const double* __restrict__ EZV, const double* __restrict__ F,
                                            const double* __restrict__ A, const double* __restrict__ F0,
                                            double* __restrict__ out, Geo g, const DevScal* __restrict__ sc,
@@ -324,7 +349,7 @@ __global__ __launch_bounds__(BS) void k_ct(const double* __restrict__ EZX, const
     double a[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-      const double e = 0.5 * wgt(q) * EZX[q * pl + o];
+      const double e = 0.5 * wgt(q) * (double)EZX[q * pl + o];
       a[1] += e * iy1;
       a[2] += e * (2 * lam(q) * iy1);
     }
@@ -340,7 +365,7 @@ __global__ __launch_bounds__(BS) void k_ct(const double* __restrict__ EZX, const
     double a[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-      const double e = 0.5 * wgt(q) * EZY[q * pl + o];
+      const double e = 0.5 * wgt(q) * (double)EZY[q * pl + o];
       a[1] -= e * ix1;
       a[2] -= e * (2 * lam(q) * ix1);
     }
@@ -387,75 +412,98 @@ __device__ __forceinline__ double glleval(const double* m, int n, int i1, int j1
 
 __device__ __forceinline__ void pp_cell(double m[NPL]) {
   const double eps = 1e-13;
-  // density
+  // density: eta = the LAST |(rbar - eps)/(rbar - rq)| below 1.  For
+  // doubles, RN(|a/b|) < 1 exactly when |a| < |b|, so only that point's
+  // quotient is formed.
   double eta = 1.0;
   const double rbar = m[PL(0, 0)];
+  const double na = rbar - eps;
+  double nb = 0.0;
+  bool any = false;
 #pragma unroll
   for (int i1 = 0; i1 < 5; i1++)
 #pragma unroll
     for (int j1 = 0; j1 < 5; j1++)
 #pragma unroll
       for (int f = 0; f < 2; f++) {
-        const double rq = glleval(m, 0, i1, j1, f);
-        const double eta1 = fabs((rbar - eps) / (rbar - rq));
-        if (eta1 < 1.0) eta = eta1;
+        const double b = rbar - glleval(m, 0, i1, j1, f);
+        if (fabs(na) < fabs(b)) { nb = b; any = true; }
       }
+  if (any) eta = fabs(na / nb);
   if (eta < 1.0) {
 #pragma unroll
     for (int b = 1; b < 6; b++) m[PL(b, 0)] = 0.5 * eta * m[PL(b, 0)];
   }
-  // pressure
-  eta = 1.0;
-  double eta1 = 1.0;
+  // pressure.  The reference keeps eta1 from the last negative-pressure
+  // point and takes eta = min(eta, eta1) at every point, i.e. eta is the
+  // minimum of eta1 over the negative-pressure points.  With the nodes the
+  // reference holds (-1, 0, 0, 0, 1) only 30 of its 50 points are distinct;
+  // their pressures are evaluated unrolled and the bisection
+  // (pp_Limiter.f90:101-125, at most 47 halvings) runs only where p < 0.
   double ubar[8];
 #pragma unroll
   for (int n = 0; n < 8; n++) ubar[n] = m[PL(0, n)];
-#pragma unroll 1
-  for (int k = 0; k < 50; k++) {
-    const int i1 = k / 10, j1 = (k / 2) % 5, f = k % 2;
+  unsigned neg = 0u;   // bit k: distinct point k has negative pressure
+#pragma unroll
+  for (int k = 0; k < 30; k++) {
+    // k < 15: family 1 (x = -1, 0, 1; y = Gauss); k >= 15: family 2
+    const int f = k / 15, a = (k % 15) / 5, b = k % 5;
+    const double x = f == 0 ? (a == 0 ? -1.0 : a == 1 ? 0.0 : 1.0) : lam(b);
+    const double y = f == 0 ? lam(b) : (a == 0 ? -1.0 : a == 1 ? 0.0 : 1.0);
     double uq[8];
 #pragma unroll
     for (int n = 0; n < 8; n++) {
-      const double x = f == 0 ? lamL(i1) : lam(i1);
-      const double y = f == 0 ? lam(j1) : lamL(j1);
-      double a = 0.0;
+      double acc = 0.0;
 #pragma unroll
-      for (int b = 0; b < 10; b++) {
-        const int p = PL(b, n);
-        if (p < 0) continue;
-        // basis values at run-time nodes: same formulas as the tables
-        double v;
-        {
-          MHD_NOCONTRACT
-          v = b == 0 ? 1.0 : b == 1 ? x : b == 2 ? y : b == 3 ? x * x - 1.0 / 3.0 : b == 4 ? x * y
-            : b == 5 ? y * y - 1.0 / 3.0 : b == 6 ? x * x * x - 3.0 * x / 5.0 : b == 7 ? (x * x - 1.0 / 3.0) * (y)
-            : b == 8 ? (x) * (y * y - 1.0 / 3.0) : y * y * y - 3.0 * y / 5.0;
-        }
-        a += m[p] * v;
+      for (int bb = 0; bb < 10; bb++) {
+        const int p = PL(bb, n);
+        if (p >= 0) acc += m[p] * phi(bb, x, y);
       }
-      uq[n] = a;
+      uq[n] = acc;
     }
-    const double pq = pressure<double>(uq);
-    if (pq < 0.0) {
-      // calculate_tq (pp_Limiter.f90:101-125): bisection, at most 47 halvings
-      double ta = 0.0, tb = 1.0, ut[8];
-      while (tb - ta > 1e-14) {
-        const double tq = 0.5 * (ta + tb);
-#pragma unroll
-        for (int n = 0; n < 8; n++) ut[n] = tq * uq[n] + (1 - tq) * ubar[n];
-        if (pressure<double>(ut) < 1e-12) tb = tq;
-        else ta = tq;
-      }
-      const double tq = ta;
-      double s1 = 0.0, s2 = 0.0;
+    if (pressure<double>(uq) < 0.0) neg |= 1u << k;
+  }
+  eta = 1.0;
+  while (neg) {
+    const int k = __builtin_ctz(neg);
+    neg &= neg - 1u;
+    const int f = k / 15, a = (k % 15) / 5, b = k % 5;
+    const double g = a == 0 ? -1.0 : a == 1 ? 0.0 : 1.0;
+    const double lb = b == 0 ? lam(0) : b == 1 ? lam(1) : b == 2 ? lam(2) : b == 3 ? lam(3) : lam(4);
+    const double x = f == 0 ? g : lb, y = f == 0 ? lb : g;
+    double uq[8];
+    {
+      MHD_NOCONTRACT
+      const double pv[10] = {1.0, x, y, x * x - 1.0 / 3.0, x * y, y * y - 1.0 / 3.0, x * x * x - 3.0 * x / 5.0,
+                             (x * x - 1.0 / 3.0) * (y), (x) * (y * y - 1.0 / 3.0), y * y * y - 3.0 * y / 5.0};
 #pragma unroll
       for (int n = 0; n < 8; n++) {
-        const double sq = tq * uq[n] + (1 - tq) * ubar[n];
-        s1 += (sq - ubar[n]) * (sq - ubar[n]);
-        s2 += (uq[n] - ubar[n]) * (uq[n] - ubar[n]);
+        double acc = 0.0;
+#pragma unroll
+        for (int bb = 0; bb < 10; bb++) {
+          const int p = PL(bb, n);
+          if (p >= 0) acc = fma(m[p], pv[bb], acc);
+        }
+        uq[n] = acc;
       }
-      eta1 = sqrt(s1) / sqrt(s2);
     }
+    double ta = 0.0, tb = 1.0, ut[8];
+    while (tb - ta > 1e-14) {
+      const double tq = 0.5 * (ta + tb);
+#pragma unroll
+      for (int n = 0; n < 8; n++) ut[n] = tq * uq[n] + (1 - tq) * ubar[n];
+      if (pressure<double>(ut) < 1e-12) tb = tq;
+      else ta = tq;
+    }
+    const double tq = ta;
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      const double sq = tq * uq[n] + (1 - tq) * ubar[n];
+      s1 += (sq - ubar[n]) * (sq - ubar[n]);
+      s2 += (uq[n] - ubar[n]) * (uq[n] - ubar[n]);
+    }
+    const double eta1 = sqrt(s1) / sqrt(s2);
     if (eta1 < eta) eta = eta1;
   }
   if (eta < 1.0) eta = 0.5 * eta;
@@ -1037,19 +1085,31 @@ int dispatch_ft(int ft, F&& f) {
   }
 }
 
-// face fluxes + face EMFs (+ vertex EMFs when with_vertex)
+// face fluxes + face EMFs (+ vertex EMFs when with_vertex); the face
+// quadrature runs in double (plain form) or float (mixed form)
 int launch_fluxes(mhd2d_ctx* ctx, const double* U, bool with_vertex, const DevScal* sc) {
   const Geo& g = ctx->g;
+  const bool mixed = ctx->cfg.precision == MHD2D_MIXED;
   return dispatch_ft(ctx->cfg.flux_type, [&](auto FTc) -> int {
     constexpr int FT = decltype(FTc)::value;
     const long long nxf = (long long)(g.Nx + 1) * g.Ny * 5, nyf = (long long)g.Nx * (g.Ny + 1) * 5;
     {
       Timed tm(ctx, KC_XFACE);
-      hipLaunchKernelGGL(k_xface<FT>, dim3(nblk(nxf)), dim3(BS), 0, ctx->stream, U, ctx->FX, ctx->EZX, g, sc);
+      if (mixed)
+        hipLaunchKernelGGL((k_xface<FT, float>), dim3(nblk(nxf)), dim3(BS), 0, ctx->stream, U, (float*)ctx->FX,
+                           (float*)ctx->EZX, g, sc);
+      else
+        hipLaunchKernelGGL((k_xface<FT, double>), dim3(nblk(nxf)), dim3(BS), 0, ctx->stream, U, ctx->FX, ctx->EZX, g,
+                           sc);
     }
     {
       Timed tm(ctx, KC_YFACE);
-      hipLaunchKernelGGL(k_yface<FT>, dim3(nblk(nyf)), dim3(BS), 0, ctx->stream, U, ctx->FY, ctx->EZY, g, sc);
+      if (mixed)
+        hipLaunchKernelGGL((k_yface<FT, float>), dim3(nblk(nyf)), dim3(BS), 0, ctx->stream, U, (float*)ctx->FY,
+                           (float*)ctx->EZY, g, sc);
+      else
+        hipLaunchKernelGGL((k_yface<FT, double>), dim3(nblk(nyf)), dim3(BS), 0, ctx->stream, U, ctx->FY, ctx->EZY, g,
+                           sc);
     }
     if (with_vertex) {
       Timed tm(ctx, KC_VERTEX);
@@ -1061,31 +1121,53 @@ int launch_fluxes(mhd2d_ctx* ctx, const double* U, bool with_vertex, const DevSc
   });
 }
 
-int launch_cell(mhd2d_ctx* ctx, int mode, const double* S, const double* A, const double* U0, double* out,
-                const DevScal* sc, double dt) {
+template <int MODE>
+void launch_cell_t(mhd2d_ctx* ctx, const double* S, const double* A, const double* U0, double* out,
+                   const DevScal* sc, double dt) {
   const Geo& g = ctx->g;
   const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
+  if (ctx->cfg.precision == MHD2D_MIXED)
+    hipLaunchKernelGGL((k_cell<MODE, float>), dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, (const float*)ctx->FX,
+                       (const float*)ctx->FY, A, U0, out, g, sc, dt);
+  else
+    hipLaunchKernelGGL((k_cell<MODE, double>), dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, (const double*)ctx->FX,
+                       (const double*)ctx->FY, A, U0, out, g, sc, dt);
+}
+
+int launch_cell(mhd2d_ctx* ctx, int mode, const double* S, const double* A, const double* U0, double* out,
+                const DevScal* sc, double dt) {
   Timed tm(ctx, KC_CELL);
   switch (mode) {
-    case 0: hipLaunchKernelGGL(k_cell<0>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, ctx->FX, ctx->FY, A, U0, out, g, sc, dt); break;
-    case 1: hipLaunchKernelGGL(k_cell<1>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, ctx->FX, ctx->FY, A, U0, out, g, sc, dt); break;
-    case 2: hipLaunchKernelGGL(k_cell<2>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, ctx->FX, ctx->FY, A, U0, out, g, sc, dt); break;
-    default: hipLaunchKernelGGL(k_cell<3>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, ctx->FX, ctx->FY, A, U0, out, g, sc, dt); break;
+    case 0: launch_cell_t<0>(ctx, S, A, U0, out, sc, dt); break;
+    case 1: launch_cell_t<1>(ctx, S, A, U0, out, sc, dt); break;
+    case 2: launch_cell_t<2>(ctx, S, A, U0, out, sc, dt); break;
+    default: launch_cell_t<3>(ctx, S, A, U0, out, sc, dt); break;
   }
   LAUNCHCHK(ctx);
   return 0;
 }
 
-int launch_ct(mhd2d_ctx* ctx, int mode, const double* F, const double* A, const double* F0, double* out,
-              const DevScal* sc, double dt) {
+template <int MODE>
+void launch_ct_t(mhd2d_ctx* ctx, const double* F, const double* A, const double* F0, double* out,
+                 const DevScal* sc, double dt) {
   const Geo& g = ctx->g;
   const long long n = (long long)(g.Nx + 1) * g.Ny + (long long)g.Nx * (g.Ny + 1);
+  if (ctx->cfg.precision == MHD2D_MIXED)
+    hipLaunchKernelGGL((k_ct<MODE, float>), dim3(nblk(n)), dim3(BS), 0, ctx->stream, (const float*)ctx->EZX,
+                       (const float*)ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt);
+  else
+    hipLaunchKernelGGL((k_ct<MODE, double>), dim3(nblk(n)), dim3(BS), 0, ctx->stream, (const double*)ctx->EZX,
+                       (const double*)ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt);
+}
+
+int launch_ct(mhd2d_ctx* ctx, int mode, const double* F, const double* A, const double* F0, double* out,
+              const DevScal* sc, double dt) {
   Timed tm(ctx, KC_CT);
   switch (mode) {
-    case 0: hipLaunchKernelGGL(k_ct<0>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, ctx->EZX, ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt); break;
-    case 1: hipLaunchKernelGGL(k_ct<1>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, ctx->EZX, ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt); break;
-    case 2: hipLaunchKernelGGL(k_ct<2>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, ctx->EZX, ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt); break;
-    default: hipLaunchKernelGGL(k_ct<3>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, ctx->EZX, ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt); break;
+    case 0: launch_ct_t<0>(ctx, F, A, F0, out, sc, dt); break;
+    case 1: launch_ct_t<1>(ctx, F, A, F0, out, sc, dt); break;
+    case 2: launch_ct_t<2>(ctx, F, A, F0, out, sc, dt); break;
+    default: launch_ct_t<3>(ctx, F, A, F0, out, sc, dt); break;
   }
   LAUNCHCHK(ctx);
   return 0;
@@ -1163,7 +1245,7 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
   const int bcs[4] = {cfg->bcR, cfg->bcL, cfg->bcU, cfg->bcD};
   for (int b : bcs)
     if (b != 1 && b != 2) return -1;
-  if (cfg->precision != MHD2D_FP64) return -4;   // only the plain form is built in this library version
+  if (cfg->precision != MHD2D_FP64 && cfg->precision != MHD2D_MIXED) return -1;
   mhd2d_ctx* ctx = (mhd2d_ctx*)calloc(1, sizeof(mhd2d_ctx));
   if (!ctx) return -1;
   new (&ctx->prof_ev) decltype(ctx->prof_ev)();

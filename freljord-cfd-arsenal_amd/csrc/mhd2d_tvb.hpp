@@ -9,12 +9,12 @@
 // eigenmatrix.f90:397-404 (sum |R| > 1e4 or sum |L| > 1e4) needs both sums
 // before any product is used, so R's columns are generated twice.
 //
-// The eigen-system scalars and matrix entries are computed without
-// contraction and with true divisions, in the reference's operation order:
-// for B parallel to n the reference's alpha_f = sqrt|a^2 - c_s^2| / ...
-// (eigenmatrix.f90:58) is the square root of a rounding residue, so a single
-// ulp of difference anywhere in it changes alpha_f by O(1) relative.  With
-// the same operations on the same inputs the results are bit-identical.
+// The eigen-system scalars are computed without contraction and with true
+// divisions, in the reference's operation order: for B parallel to n the
+// reference's alpha_f = sqrt|a^2 - c_s^2| / ... (eigenmatrix.f90:58) is the
+// square root of a rounding residue, so a single ulp of difference in it
+// changes alpha_f by O(1) relative.  The matrix entries built from those
+// scalars are well conditioned and use reciprocals and FMA.
 #pragma once
 #include "mhd2d_physics.hpp"
 #include "mhd2d_tables.hpp"
@@ -30,6 +30,8 @@ struct EigScal {
   double rhosqL, a2L, aL, cfL, alphafL, alphasL, BN, BT, gu1sq, sqgam, nen, nen2, nen3, nen31, nen51, nen52, nen71;
   double Term51, Term52, Term53, Term54, Term55, Term56, Term57;
   double Term71, Term72, Term73, Term74, Term75, Term76, Term77;
+  // reciprocals shared by the matrix entries
+  double irho, inen2, inen3, inen31, inen51, inen52, inen71, irc;
 };
 
 // compute_t_vector (eigenmatrix.f90:411-433) + pick_orthogonal (:451-469)
@@ -155,11 +157,18 @@ __device__ __forceinline__ void eig_scalars(const double U[8], EigScal& s) {
   s.Term77 = -alphaf * (-n1 * n3 * t1 + (n1 * n1) * t3 + n2 * (-n3 * t2 + n2 * t3));
   s.nen71 = twosq * beta1 * nen * gu1sq *
             (a * (BN * BN) * (alphas * alphas) + rhosq * (cf * cf) * alphaf * (a * rhosq * alphaf + BT * alphas));
+  s.irho = 1.0 / rho;
+  s.inen2 = 1.0 / s.nen2;
+  s.inen3 = 1.0 / s.nen3;
+  s.inen31 = 1.0 / s.nen31;
+  s.inen51 = 1.0 / s.nen51;
+  s.inen52 = 1.0 / s.nen52;
+  s.inen71 = 1.0 / s.nen71;
+  s.irc = 1.0 / (s.rhosqR * s.cfR);
 }
 
 // row m (1..8) of the conservative left eigenvectors evl(m, 1:8)
 __device__ __forceinline__ void eig_lrow(const EigScal& s, int m, double e[8]) {
-  MHD_NOCONTRACT
   const double n1 = s.n1, n2 = s.n2, n3 = s.n3, t1 = s.t1, t2 = s.t2, t3 = s.t3;
   const double TxN1 = n3 * t2 - n2 * t3, TxN2 = n1 * t3 - n3 * t1, TxN3 = n2 * t1 - n1 * t2;
   double l[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -167,45 +176,44 @@ __device__ __forceinline__ void eig_lrow(const EigScal& s, int m, double e[8]) {
     l[0] = 1.0 / (s.sqgam * s.rhosqL);
     l[4] = -1.0 / (s.a2L * s.sqgam * s.rhosqL);
   } else if (m == 2) {
-    l[5] = (-n2 * t1 * t2 - n3 * t1 * t3 + n1 * (t2 * t2 + t3 * t3)) / s.nen2;
-    l[6] = (-t2 * (n1 * t1 + n3 * t3) + n2 * (t1 * t1 + t3 * t3)) / s.nen2;
-    l[7] = (n3 * (t1 * t1 + t2 * t2) - (n1 * t1 + n2 * t2) * t3) / s.nen2;
+    l[5] = (-n2 * t1 * t2 - n3 * t1 * t3 + n1 * (t2 * t2 + t3 * t3)) * s.inen2;
+    l[6] = (-t2 * (n1 * t1 + n3 * t3) + n2 * (t1 * t1 + t3 * t3)) * s.inen2;
+    l[7] = (n3 * (t1 * t1 + t2 * t2) - (n1 * t1 + n2 * t2) * t3) * s.inen2;
   } else if (m == 3 || m == 4) {
     const double sg = m == 3 ? 1.0 : -1.0;
-    l[1] = sg * (s.rho * (-TxN1) / s.nen3);
-    l[2] = sg * (s.rho * (-TxN2) / s.nen3);
-    l[3] = sg * (s.rho * (-TxN3) / s.nen3);
-    l[5] = TxN1 / s.nen31; l[6] = TxN2 / s.nen31; l[7] = TxN3 / s.nen31;
+    l[1] = sg * (s.rho * (-TxN1) * s.inen3);
+    l[2] = sg * (s.rho * (-TxN2) * s.inen3);
+    l[3] = sg * (s.rho * (-TxN3) * s.inen3);
+    l[5] = TxN1 * s.inen31; l[6] = TxN2 * s.inen31; l[7] = TxN3 * s.inen31;
   } else if (m == 5 || m == 6) {
     const double sg = m == 5 ? 1.0 : -1.0;
-    l[1] = sg * (-s.Term51 / s.nen51);
-    l[2] = sg * (s.Term52 / s.nen51);
-    l[3] = sg * (s.Term53 / s.nen51);
+    l[1] = sg * (-s.Term51 * s.inen51);
+    l[2] = sg * (s.Term52 * s.inen51);
+    l[3] = sg * (s.Term53 * s.inen51);
     l[4] = s.Term54;
-    l[5] = s.Term55 / s.nen52; l[6] = s.Term56 / s.nen52; l[7] = s.Term57 / s.nen52;
+    l[5] = s.Term55 * s.inen52; l[6] = s.Term56 * s.inen52; l[7] = s.Term57 * s.inen52;
   } else {
     const double sg = m == 7 ? 1.0 : -1.0;
-    l[1] = sg * (s.Term71 / s.nen71);
-    l[2] = sg * (s.Term72 / s.nen71);
-    l[3] = sg * (s.Term73 / s.nen71);
+    l[1] = sg * (s.Term71 * s.inen71);
+    l[2] = sg * (s.Term72 * s.inen71);
+    l[3] = sg * (s.Term73 * s.inen71);
     l[4] = s.Term74;
-    l[5] = s.Term75 / s.nen52; l[6] = s.Term76 / s.nen52; l[7] = s.Term77 / s.nen52;
+    l[5] = s.Term75 * s.inen52; l[6] = s.Term76 * s.inen52; l[7] = s.Term77 * s.inen52;
   }
-  const double g1 = s.g1, rho = s.rho, u1 = s.u1, u2 = s.u2, u3 = s.u3;
-  e[0] = l[0] * g1 - l[1] * u1 * g1 / rho - l[2] * u2 * g1 / rho - l[3] * u3 * g1 / rho +
-         l[4] * (g1 * g1) * (u1 * u1 + u2 * u2 + u3 * u3) * .5;
-  e[1] = -l[4] * u1 * (g1 * g1) + l[1] * g1 / rho;
-  e[2] = -l[4] * u2 * (g1 * g1) + l[2] * g1 / rho;
-  e[3] = -l[4] * u3 * (g1 * g1) + l[3] * g1 / rho;
-  e[4] = l[4] * (g1 * g1);
-  e[5] = l[5] * g1 - s.B1 * l[4] * (g1 * g1);
-  e[6] = l[6] * g1 - s.B2 * l[4] * (g1 * g1);
-  e[7] = l[7] * g1 - s.B3 * l[4] * (g1 * g1);
+  const double g1 = s.g1, g12 = g1 * g1, gr = g1 * s.irho;
+  const double u1 = s.u1, u2 = s.u2, u3 = s.u3;
+  e[0] = l[0] * g1 - (l[1] * u1 + l[2] * u2 + l[3] * u3) * gr + l[4] * g12 * (u1 * u1 + u2 * u2 + u3 * u3) * .5;
+  e[1] = -l[4] * u1 * g12 + l[1] * gr;
+  e[2] = -l[4] * u2 * g12 + l[2] * gr;
+  e[3] = -l[4] * u3 * g12 + l[3] * gr;
+  e[4] = l[4] * g12;
+  e[5] = l[5] * g1 - s.B1 * l[4] * g12;
+  e[6] = l[6] * g1 - s.B2 * l[4] * g12;
+  e[7] = l[7] * g1 - s.B3 * l[4] * g12;
 }
 
 // column m (1..8) of the conservative right eigenvectors evr(1:8, m)
 __device__ __forceinline__ void eig_rcol(const EigScal& s, int m, double e[8]) {
-  MHD_NOCONTRACT
   const double n1 = s.n1, n2 = s.n2, n3 = s.n3, t1 = s.t1, t2 = s.t2, t3 = s.t3;
   const double TxN1 = n3 * t2 - n2 * t3, TxN2 = n1 * t3 - n3 * t1, TxN3 = n2 * t1 - n1 * t2;
   const double a = s.aR, a2 = s.a2R, rhosq = s.rhosqR, sqg2 = s.sqg2, cf = s.cfR;
@@ -217,38 +225,40 @@ __device__ __forceinline__ void eig_rcol(const EigScal& s, int m, double e[8]) {
     r[5] = s.sq1og * a * n1; r[6] = s.sq1og * a * n2; r[7] = s.sq1og * a * n3;
   } else if (m == 3 || m == 4) {
     const double sg = m == 3 ? 1.0 : -1.0;
-    r[1] = sg * (-s.sq12 * (s.sqpr * (TxN1)));
-    r[2] = sg * (-s.sq12 * (s.sqpr * (TxN2)));
-    r[3] = sg * (-s.sq12 * (s.sqpr * (TxN3)));
-    r[5] = s.sq12 * s.sqpor * (TxN1); r[6] = s.sq12 * s.sqpor * (TxN2); r[7] = s.sq12 * s.sqpor * (TxN3);
+    r[1] = sg * (-s.sq12 * (s.sqpr * TxN1));
+    r[2] = sg * (-s.sq12 * (s.sqpr * TxN2));
+    r[3] = sg * (-s.sq12 * (s.sqpr * TxN3));
+    r[5] = s.sq12 * s.sqpor * TxN1; r[6] = s.sq12 * s.sqpor * TxN2; r[7] = s.sq12 * s.sqpor * TxN3;
   } else if (m == 5 || m == 6) {
     const double sg = m == 5 ? 1.0 : -1.0;
+    const double c = sqg2 * s.irc;
     r[0] = sqg2 * alphaf * rhosq;
-    r[1] = sg * (sqg2 * ((alphaf * a2 * n1 + alphas * a * ((bst)*n1 - (bN)*t1))) / (rhosq * cf));
-    r[2] = sg * (sqg2 * ((alphaf * a2 * n2 + alphas * a * ((bst)*n2 - (bN)*t2))) / (rhosq * cf));
-    r[3] = sg * (sqg2 * ((alphaf * a2 * n3 + alphas * a * ((bst)*n3 - (bN)*t3))) / (rhosq * cf));
+    r[1] = sg * (c * (alphaf * a2 * n1 + alphas * a * (bst * n1 - bN * t1)));
+    r[2] = sg * (c * (alphaf * a2 * n2 + alphas * a * (bst * n2 - bN * t2)));
+    r[3] = sg * (c * (alphaf * a2 * n3 + alphas * a * (bst * n3 - bN * t3)));
     r[4] = sqg2 * alphaf * rhosq * a2;
     r[5] = sqg2 * alphas * a * t1; r[6] = sqg2 * alphas * a * t2; r[7] = sqg2 * alphas * a * t3;
   } else {
     const double sg = m == 7 ? 1.0 : -1.0;
-    const double b1 = s.beta1R;
+    const double c = s.beta1R * sqg2 * s.irc;
     r[0] = sqg2 * alphas * rhosq;
-    r[1] = sg * (b1 * sqg2 * (alphaf * (cf * cf) * t1 + a * n1 * alphas * (bN)) / (rhosq * cf));
-    r[2] = sg * (b1 * sqg2 * (alphaf * (cf * cf) * t2 + alphas * a * (bN)*n2) / (rhosq * cf));
-    r[3] = sg * (b1 * sqg2 * (alphaf * (cf * cf) * t3 + alphas * a * (bN)*n3) / (rhosq * cf));
+    r[1] = sg * (c * (alphaf * (cf * cf) * t1 + a * n1 * alphas * bN));
+    r[2] = sg * (c * (alphaf * (cf * cf) * t2 + alphas * a * bN * n2));
+    r[3] = sg * (c * (alphaf * (cf * cf) * t3 + alphas * a * bN * n3));
     r[4] = (a * a) * sqg2 * alphas * rhosq;
     r[5] = -sqg2 * alphaf * a * t1; r[6] = -sqg2 * alphaf * a * t2; r[7] = -sqg2 * alphaf * a * t3;
   }
-  const double g1 = s.g1, rho = s.rho, u1 = s.u1, u2 = s.u2, u3 = s.u3;
-  e[0] = r[0] / g1;
-  e[1] = (r[0] * u1 + r[1] * rho) / g1;
-  e[2] = (r[0] * u2 + r[2] * rho) / g1;
-  e[3] = (r[0] * u3 + r[3] * rho) / g1;
-  e[4] = (r[4] / g1 + s.B1 * r[5] + s.B2 * r[6] + s.B3 * r[7] + 0.5 * r[0] * (u1 * u1 + u2 * u2 + u3 * u3) +
-          r[1] * u1 * rho + r[2] * u2 * rho + r[3] * u3 * rho) / g1;
-  e[5] = r[5] / g1;
-  e[6] = r[6] / g1;
-  e[7] = r[7] / g1;
+  constexpr double ig1 = 1.0 / (5.0 / 3.0 - 1.0);
+  const double rho = s.rho, u1 = s.u1, u2 = s.u2, u3 = s.u3;
+  e[0] = r[0] * ig1;
+  e[1] = (r[0] * u1 + r[1] * rho) * ig1;
+  e[2] = (r[0] * u2 + r[2] * rho) * ig1;
+  e[3] = (r[0] * u3 + r[3] * rho) * ig1;
+  e[4] = (r[4] * ig1 + s.B1 * r[5] + s.B2 * r[6] + s.B3 * r[7] + 0.5 * r[0] * (u1 * u1 + u2 * u2 + u3 * u3) +
+          r[1] * u1 * rho + r[2] * u2 * rho + r[3] * u3 * rho) * ig1;
+  e[5] = r[5] * ig1;
+  e[6] = r[6] * ig1;
+  e[7] = r[7] * ig1;
 }
 
 // minmod.f90:5-27 for one characteristic field
