@@ -27,21 +27,51 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 DOF_PER_CELL = 48
 
-# algorithmic bytes per interior cell for one launch of each kernel class,
-# FP64 state of 52 planes (416 B/cell); see DESIGN.md "Kernels and rooflines"
-ALG_BYTES_FP64 = {
-    "xface": 416 + 45 * 8,          # state in, 40 fluxes + 5 face EMFs out
-    "yface": 416 + 45 * 8,
-    "vertex": 416 + 8,              # state in, one vertex EMF out
-    "cell": 416 + 640 + 416 + (0 + 832 + 832) / 3.0,   # state + fluxes in, stage value out, RK inputs
-    "ct": 2 * 40 + 8 + 48 + 48,     # face EMFs, vertex EMF, faces in and out
-    "limit": 416 + 48 + 416,        # stage value + faces in, limited state out
-    "setbc": 0.0,
-    "dt": 64,
-    "kxrcf": 416,
-    "diag": 416,
-}
+# algorithmic bytes per interior cell for one launch of each kernel class.
+# State: 52 FP64 planes (416 B/cell) in both forms; face B: 6 FP64 planes
+# (48 B/cell).  Face fluxes (40 values per face), face EMFs (5 per face)
+# are FP64 in the plain form and FP32 in the mixed form (bytes B below).
+# One x face and one y face belong to each cell.
+def alg_bytes(form):
+    B = 8 if form == "fp64" else 4
+    return {
+        "xface": 416 + 45 * B,          # state in; 40 fluxes + 5 face EMFs out
+        "yface": 416 + 45 * B,
+        "vertex": 416 + 8,              # state in; one FP64 vertex EMF out
+        # state + x/y fluxes in, stage value out; the RK inputs average
+        # (0 + 2 + 2) / 3 state reads (stages 2 and 3 read U0 and the
+        # unlimited previous stage value)
+        "cell": 416 + 2 * 40 * B + 416 + 4 * 416 / 3.0,
+        "ct": 2 * 5 * B + 8 + 48 + 48 + 4 * 48 / 3.0,   # face + vertex EMFs, faces in/out, RK inputs
+        "limit": 416 + 48 + 416,        # stage value + faces in, limited state out
+        "setbc": 0.0,
+        "dt": 64,                       # 8 cell averages
+        "kxrcf": 416,
+        "diag": 416,
+    }
+
+
 STEP_ALG_BYTES_PER_CELL = 2688     # 42 independent FP64 values x 8 state transfers per SSP-RK3 step
+# kernel-name prefix of each class in the rocprofv3 traces
+KERNEL_SYMBOL = {"xface": "k_xface", "yface": "k_yface", "vertex": "k_vertex", "cell": "k_cell", "ct": "k_ct",
+                 "limit": "k_limit", "dt": "k_dt", "kxrcf": "k_kxrcf", "diag": "k_diag", "setbc": "k_setbc"}
+
+
+def measured_traffic(kernel, form, N):
+    """HBM bytes per launch of one kernel class, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs, scaled by the
+    same counters on a plain streaming kernel); None if not measured."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_ot{N}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) >= 5 and cols[0] == KERNEL_SYMBOL.get(kernel) and cols[1] == form:
+            try:
+                return float(cols[4])
+            except ValueError:
+                return None
+    return None
 
 
 def import_pkg():
@@ -79,6 +109,34 @@ def cpu_baseline(N=256, steps=2):
                       f"{N}x{N}, {steps} RK3 steps (CFL 0.15 as shipped), {secs:.1f} s"}
 
 
+def counters_run(pkg, N, dom):
+    """Calibration dispatch plus two steps of each form, for rocprofv3 --pmc.
+
+    The calibration kernel is k_axpy of the per-subroutine stage update
+    (mhd2d_rk_stage 1 with dt = 0): over the 52 state planes it reads two
+    arrays and writes one, n = 52 * plane doubles each, so its FETCH_SIZE
+    and WRITE_SIZE must read 16 n and 8 n bytes; the ratio of measured to
+    known bytes scales the counters of the other kernels."""
+    s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1)
+    s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
+    s.rk_stage(1, 0.0)
+    s.sync()
+    P = (N + 2 + 15 + 15) // 16 * 16
+    n = 52 * (N + 2) * P
+    print(f"[counters] k_axpy over {n} doubles: reads {16 * n} B, writes {8 * n} B", file=sys.stderr, flush=True)
+    s.close()
+    for name, prec in (("fp64", pkg.FP64), ("mixed", pkg.MIXED)):
+        s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=pkg.IND_ALL,
+                       precision=prec)
+        s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
+        s.rk3_prologue()
+        s.step()
+        s.step()
+        s.sync()
+        s.close()
+        print(f"[counters] {name}: two steps at {N}^2 done", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +147,9 @@ def main():
     ap.add_argument("--error", type=int, default=1, help="report mixed-vs-FP64 L2 errors (downloads both states)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--counters", action="store_true",
+                    help="no timing: the dispatches a rocprofv3 --pmc pass reads (a streaming kernel of known "
+                         "size, then two steps of each form)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no sharding, no collectives); use --gpus 1",
@@ -98,6 +159,9 @@ def main():
     pkg = import_pkg()
     N = args.n
     dom = (0.0, 2 * np.pi, 0.0, 2 * np.pi)
+    if args.counters:
+        counters_run(pkg, N, dom)
+        return
 
     def run(prec, steps, warmup, profile_steps):
         s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=pkg.IND_ALL,
@@ -150,7 +214,7 @@ def main():
     dom_k = max(prof, key=lambda k: prof[k][0])
     ms_tot, launches = prof[dom_k]
     avg_ms = ms_tot / max(launches, 1)
-    bytes_launch = ALG_BYTES_FP64[dom_k] * N * N
+    bytes_launch = alg_bytes(args.precision)[dom_k] * N * N
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     step_gbs = STEP_ALG_BYTES_PER_CELL * N * N / (ms_per_step * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline()
@@ -175,7 +239,8 @@ def main():
                    "speedup_vs_fp64": value / results["fp64"]["value"],
                    "rel_l2_vs_fp64": errors},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom_k,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_k, args.precision, N),
+                     "kernel": dom_k, "alg_bytes_per_launch": bytes_launch,
                      "kernel_avg_ms": avg_ms, "step_achieved": step_gbs,
                      "step_frac": step_gbs / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
