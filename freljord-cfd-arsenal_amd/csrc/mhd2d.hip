@@ -17,603 +17,16 @@
 //   k_ct<MODE>         one lane per face: CT right-hand side + combination
 //   k_limit            one lane per cell: TVB (+ positivity) + Balsara
 //                      divergence-free reconstruction
-//   k_setbc            halo ring
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <new>
-#include <utility>
-#include <vector>
-
-#include "../../include/mhd2d.h"
-#include "mhd2d_physics.hpp"
-#include "mhd2d_tables.hpp"
-#include "mhd2d_tvb.hpp"
+//   k_setbc            halo ring//
+// This file: halo ring, reductions, initial conditions and the C ABI.
+// Kernels of the stage live in mhd2d_faces.hip, mhd2d_cell.hip and
+// mhd2d_limit.hip; shared declarations in mhd2d_internal.hpp.
+#include "mhd2d_internal.hpp"
 
 using namespace mhd;
+using namespace mhd2d_impl;
 
 namespace {
-
-constexpr int BS = 256;
-
-// device scalars of the time loop (COMMON /time/ and the diagnostics)
-struct DevScal {
-  double t, dt, tend, CFL;
-  double alphax, alphay, umax, pmin, totaldiv;
-  unsigned long long ax_bits, ay_bits, umax_bits;
-  long long pmin_key;
-  long long count;
-  long long nflag;
-  int active;
-  int pad;
-};
-
-__device__ __forceinline__ long long dkey(double x) {   // order-preserving key of a double
-  long long b = __double_as_longlong(x);
-  return b >= 0 ? b : b ^ 0x7fffffffffffffffLL;
-}
-__device__ __forceinline__ double dunkey(long long k) {
-  return __longlong_as_double(k >= 0 ? k : k ^ 0x7fffffffffffffffLL);
-}
-
-// blocks of consecutive ids go to the same XCD (round-robin dispatch over 8)
-__device__ __forceinline__ long long xcd_block() {
-  const unsigned b = blockIdx.x, n = gridDim.x;
-  const unsigned full = n & ~7u;
-  if (b >= full) return b;
-  return (long long)(b & 7u) * (full >> 3) + (b >> 3);
-}
-
-// select one of five compile-time node values by a lane-varying index
-__device__ __forceinline__ double sel5(int q, double a0, double a1, double a2, double a3, double a4) {
-  double r = a0;
-  r = q == 1 ? a1 : r;
-  r = q == 2 ? a2 : r;
-  r = q == 3 ? a3 : r;
-  r = q == 4 ? a4 : r;
-  return r;
-}
-
-// trace of all 8 variables of one cell on one side at node q, in type T
-template <int SIDE, typename T>
-__device__ __forceinline__ void trace_at(const double* __restrict__ U, long long plane, long long c, int q,
-                                         T out[8]) {
-  constexpr double L0 = lam(0), L1 = lam(1), L2 = lam(2), L3 = lam(3), L4 = lam(4);
-  const double s = sel5(q, L0, L1, L2, L3, L4);
-  const double p2 = sel5(q, P2(L0), P2(L1), P2(L2), P2(L3), P2(L4));
-  const double p3 = sel5(q, P3(L0), P3(L1), P3(L2), P3(L3), P3(L4));
-  const double t23 = sel5(q, T23(L0), T23(L1), T23(L2), T23(L3), T23(L4));
-#pragma unroll
-  for (int n = 0; n < 8; n++) {
-    T acc = T(0);
-#pragma unroll
-    for (int b = 0; b < 10; b++) {
-      const int p = PL(b, n);
-      if (p >= 0) acc += T(U[p * plane + c]) * T(phiTr(SIDE, b, s, p2, p3, t23));
-    }
-    out[n] = acc;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// x faces: Lh.f90:72-136 (traces, fluxes), :206-234 (Riemann),
-// LhEz.f90:6-16 (face EMF).  Face i (0..Nx) of row j (1..Ny) lies between
-// cells i and i+1.
-template <int FT, typename T>
-__global__ __launch_bounds__(BS) void k_xface(const double* __restrict__ U, T* __restrict__ FX,
-                                              T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
-  const long long t = xcd_block() * BS + threadIdx.x;
-  const long long nf = (long long)(g.Nx + 1) * g.Ny;
-  if (t >= nf * 5) return;
-  const int q = (int)(t % 5);
-  const long long f = t / 5;
-  const int i = (int)(f % (g.Nx + 1)), j = 1 + (int)(f / (g.Nx + 1));
-  T UL[8], UR[8], FL[8], FR[8], F[8];
-  trace_at<0, T>(U, g.plane, cix(g, i, j), q, UL);       // right trace of cell i
-  trace_at<1, T>(U, g.plane, cix(g, i + 1, j), q, UR);   // left trace of cell i+1
-  lh_flux<T, 1>(UL, FL);
-  lh_flux<T, 1>(UR, FR);
-  T a1, b1, a2, b2;
-  eigenvalueMm<T, 1>(UL, a1, b1);
-  eigenvalueMm<T, 1>(UR, a2, b2);
-  const T SR = tmax(a1, a2), SL = tmin(b1, b2);
-  riemann<T, 1, FT>(SR, SL, UL, UR, FL, FR, F);
-  const long long o = cix(g, i, j);
-#pragma unroll
-  for (int n = 0; n < 8; n++) FX[(q * 8 + n) * g.plane + o] = F[n];
-  const T S = tmax(fabs(UL[1] / UL[0]), fabs(UR[1] / UR[0]));
-  EZ[q * g.plane + o] = T(-0.5) * (FL[6] + FR[6] - S * (UR[6] - UL[6]));
-}
-
-// y faces: Lh.f90:139-203, :237-264; LhEz.f90:18-28.  Face j (0..Ny) of
-// column i (1..Nx) lies between cells j and j+1.
-template <int FT, typename T>
-__global__ __launch_bounds__(BS) void k_yface(const double* __restrict__ U, T* __restrict__ FY,
-                                              T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
-  const long long t = xcd_block() * BS + threadIdx.x;
-  const long long nf = (long long)g.Nx * (g.Ny + 1);
-  if (t >= nf * 5) return;
-  const int q = (int)(t % 5);
-  const long long f = t / 5;
-  const int i = 1 + (int)(f % g.Nx), j = (int)(f / g.Nx);
-  T UD[8], UU[8], FD[8], FU[8], F[8];
-  trace_at<2, T>(U, g.plane, cix(g, i, j), q, UD);       // upper trace of cell j (reference UU)
-  trace_at<3, T>(U, g.plane, cix(g, i, j + 1), q, UU);   // lower trace of cell j+1 (reference UD)
-  lh_flux<T, 2, false>(UD, FD);
-  lh_flux<T, 2, true>(UU, FU);
-  T a1, b1, a2, b2;
-  eigenvalueMm<T, 2>(UD, a1, b1);
-  eigenvalueMm<T, 2>(UU, a2, b2);
-  const T SR = tmax(a1, a2), SL = tmin(b1, b2);
-  riemann<T, 2, FT>(SR, SL, UD, UU, FD, FU, F);
-  const long long o = cix(g, i, j);
-#pragma unroll
-  for (int n = 0; n < 8; n++) FY[(q * 8 + n) * g.plane + o] = F[n];
-  const T S = tmax(fabs(UD[2] / UD[0]), fabs(UU[2] / UU[0]));
-  EZ[q * g.plane + o] = T(0.5) * (FD[5] + FU[5] - S * (UU[5] - UD[5]));
-}
-
-// vertex (corner) EMF: LhEz.f90:33-69.  Vertex (i, j) is the upper-right
-// corner of cell (i, j), i = 0..Nx, j = 0..Ny.
-template <int V>
-__device__ __forceinline__ void vertex_state(const double* __restrict__ U, long long plane, long long c, double out[8]) {
-#pragma unroll
-  for (int n = 0; n < 8; n++) {
-    double acc = 0.0;
-#pragma unroll
-    for (int b = 0; b < 10; b++) {
-      const int p = PL(b, n);
-      if (p >= 0) acc += U[p * plane + c] * phiV(V, b);
-    }
-    out[n] = acc;
-  }
-}
-
-template <int FT>
-__global__ __launch_bounds__(BS) void k_vertex(const double* __restrict__ U, double* __restrict__ EZV, Geo g,
-                                               const DevScal* __restrict__ sc) {
-  const long long t = xcd_block() * BS + threadIdx.x;
-  const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
-  if (t >= nv) return;
-  const int i = (int)(t % (g.Nx + 1)), j = (int)(t / (g.Nx + 1));
-  double URU[8], ULU[8], URD[8], ULD[8];
-  vertex_state<3>(U, g.plane, cix(g, i + 1, j + 1), URU);  // LD corner of the upper-right cell
-  vertex_state<2>(U, g.plane, cix(g, i, j + 1), ULU);      // RD corner of the upper-left cell
-  vertex_state<1>(U, g.plane, cix(g, i + 1, j), URD);      // LU corner of the lower-right cell
-  vertex_state<0>(U, g.plane, cix(g, i, j), ULD);          // RU corner of the lower-left cell
-  EZV[cix(g, i, j)] = vertex_emf<double, FT>(URU, ULU, URD, ULD);
-}
-
-// ---------------------------------------------------------------------------
-// cell residual (Lh.f90:11-68 volume, :268-304 surface + mass inverse) and
-// the RK3 combination of RK3.f90 as the epilogue:
-//   MODE 0: du                     (Lh alone)
-//   MODE 1: out = S + dt du        (uI,  RK3.f90:157)
-//   MODE 2: out = 3/4 U0 + 1/4 (UI + dt du)   (RK3.f90:183,187)
-//   MODE 3: out = 1/3 U0 + 2/3 (UII + dt du)  (RK3.f90:208-210)
-// The launch covers the halo ring too: there du = 0 (Lh.f90:5), so the
-// combination carries the start-of-step halos exactly as the reference's
-// whole-array statements do (the stale halos TVB later reads).
-//
-// T is the quadrature type (double: plain form; float: mixed form).  The
-// cell-average (mode 1) residual is accumulated in double in both forms.
-// The volume integral is sum-factorised row by row in eta: per row the
-// modes collapse to 1D coefficients in xi, and the modes are re-read for
-// every row (L1 hits) instead of being held, which keeps the accumulators
-// (52 per cell) and the row state inside the register file.
-template <typename T>
-__device__ __forceinline__ T ld(const double* __restrict__ p) { return (T)(*p); }
-
-template <int MODE, typename T>
-__global__ __launch_bounds__(BS) void k_cell(const double* __restrict__ S, const T* __restrict__ FX,
-                                             const T* __restrict__ FY, const double* __restrict__ A,
-                                             const double* __restrict__ U0, double* __restrict__ out, Geo g,
-                                             const DevScal* __restrict__ sc, double dt_arg) {
-  const long long t = xcd_block() * BS + threadIdx.x;
-  const int W = g.Nx + 2;
-  if (t >= (long long)W * (g.Ny + 2)) return;
-  const int i = (int)(t % W), j = (int)(t / W);
-  const long long c = cix(g, i, j);
-  const long long pl = g.plane;
-  const double dt = sc ? sc->dt : dt_arg;
-  const bool interior = i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny;
-  double acc0[8];       // mode-1 (cell average) residuals
-  T acc[NPL];           // higher modes (slots PL(0,n) unused)
-#pragma unroll
-  for (int n = 0; n < 8; n++) acc0[n] = 0.0;
-#pragma unroll
-  for (int p = 0; p < NPL; p++) acc[p] = T(0);
-  if (interior) {
-    const T ihx1 = T(1.0 / g.hx1), ihy1 = T(1.0 / g.hy1);
-#pragma unroll
-    for (int j1 = 0; j1 < 5; j1++) {
-      const double y = lam(j1);
-      const T ty = T(y), tP2y = T(P2(y)), tP3y = T(P3(y));
-      long long cj = c;
-      asm volatile("" : "+v"(cj));   // re-read the modes for every row
-      // U(x, y_j) = c0 + c1 x + c2 P2(x) + c3 P3(x) per variable
-      T c0[8], c1[8], c2[8], c3b = T(0);
-#pragma unroll
-      for (int n = 0; n < 8; n++) {
-        const T m0 = ld<T>(S + PL(0, n) * pl + cj), m1 = ld<T>(S + PL(1, n) * pl + cj);
-        const T m2 = ld<T>(S + PL(2, n) * pl + cj), m3 = ld<T>(S + PL(3, n) * pl + cj);
-        const T m4 = ld<T>(S + PL(4, n) * pl + cj), m5 = ld<T>(S + PL(5, n) * pl + cj);
-        c0[n] = m0 + m2 * ty + m5 * tP2y;
-        c1[n] = m1 + m4 * ty;
-        c2[n] = m3;
-        if (n == 5) {   // Bx: phi7 = P3(xi), phi9 = xi P2(eta)
-          c1[n] += ld<T>(S + PL(8, 5) * pl + cj) * tP2y;
-          c3b = ld<T>(S + PL(6, 5) * pl + cj);
-        }
-        if (n == 6) {   // By: phi8 = P2(xi) eta, phi10 = P3(eta)
-          c2[n] += ld<T>(S + PL(7, 6) * pl + cj) * ty;
-          c0[n] += ld<T>(S + PL(9, 6) * pl + cj) * tP3y;
-        }
-      }
-#pragma unroll
-      for (int i1 = 0; i1 < 5; i1++) {
-        const double x = lam(i1);
-        const T tx = T(x), tP2x = T(P2(x)), tP3x = T(P3(x));
-        T Uq[8], Fx[8], Fy[8];
-#pragma unroll
-        for (int n = 0; n < 8; n++) Uq[n] = c0[n] + c1[n] * tx + c2[n] * tP2x;
-        Uq[5] += c3b * tP3x;
-        lh_flux_xy<T>(Uq, Fx, Fy);
-#pragma unroll
-        for (int n = 0; n < 8; n++) { Fx[n] *= ihx1; Fy[n] *= ihy1; }
-        const double ww = 0.25 * wgt(i1) * wgt(j1);
-#pragma unroll
-        for (int n = 0; n < 8; n++) {
-#pragma unroll
-          for (int b = 1; b < 10; b++) {
-            const int p = PL(b, n);
-            if (p < 0) continue;
-            const double cx = ww * dphix(b, x, y), cy = ww * dphiy(b, x, y);
-            if (cx != 0.0) acc[p] += T(cx) * Fx[n];
-            if (cy != 0.0) acc[p] += T(cy) * Fy[n];
-          }
-        }
-      }
-    }
-    // surface integrals
-    const T sx = T(0.5 / g.hx), sy = T(0.5 / g.hy);
-    const double sxd = 0.5 / g.hx, syd = 0.5 / g.hy;
-    const long long cL = cix(g, i - 1, j), cD = cix(g, i, j - 1);
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-      const double s = lam(q), p2 = P2(s), p3 = P3(s), t23 = T23(s);
-      const double wq = wgt(q);
-#pragma unroll
-      for (int n = 0; n < 8; n++) {
-        const T fr = FX[(q * 8 + n) * pl + c], fl = FX[(q * 8 + n) * pl + cL];
-        const T fu = FY[(q * 8 + n) * pl + c], fd = FY[(q * 8 + n) * pl + cD];
-        acc0[n] -= sxd * wq * ((double)fr - (double)fl);
-        acc0[n] -= syd * wq * ((double)fu - (double)fd);
-#pragma unroll
-        for (int b = 1; b < 10; b++) {
-          const int p = PL(b, n);
-          if (p < 0) continue;
-          acc[p] -= sx * T(wq) * (fr * T(phiTr(0, b, s, p2, p3, t23)) - fl * T(phiTr(1, b, s, p2, p3, t23)));
-          acc[p] -= sy * T(wq) * (fu * T(phiTr(2, b, s, p2, p3, t23)) - fd * T(phiTr(3, b, s, p2, p3, t23)));
-        }
-      }
-    }
-  }
-  // epilogue: mass inverse + RK combination, in double
-#pragma unroll
-  for (int p = 0; p < NPL; p++) {
-    const long long o = p * pl + c;
-    const bool avg = p < 48 && (p % 6) == 0;
-    int b = p < 48 ? p % 6 : (p == 48 ? 6 : p == 49 ? 8 : p == 50 ? 7 : 9);
-    const double du = avg ? acc0[p / 6] : (double)acc[p] * (1.0 / mmass(b));
-    double r;
-    if (MODE == 0) r = du;
-    else if (MODE == 1) r = S[o] + dt * du;
-    else if (MODE == 2) r = (3.0 / 4.0) * U0[o] + (1.0 / 4.0) * (A[o] + dt * du);
-    else r = (1.0 / 3.0) * U0[o] + (2.0 / 3.0) * (A[o] + dt * du);
-    out[o] = r;
-  }
-}
-
-// constrained transport: LhEz.f90:71-96 (+ the face combinations of
-// RK3.f90:159-212 for MODE 1..3 as in k_cell).  Face planes: 0..2 Bx modes
-// on x faces (i = 0..Nx, j = 1..Ny), 3..5 By modes on y faces (i = 1..Nx,
-// j = 0..Ny).
-template <int MODE, typename T>
-__global__ __launch_bounds__(BS) void k_ct(const T* __restrict__ EZX, const T* __restrict__ EZY,
-                                           const double* __restrict__ EZV, const double* __restrict__ F,
-                                           const double* __restrict__ A, const double* __restrict__ F0,
-                                           double* __restrict__ out, Geo g, const DevScal* __restrict__ sc,
-                                           double dt_arg) {
-  const long long t = xcd_block() * BS + threadIdx.x;
-  const long long nx = (long long)(g.Nx + 1) * g.Ny, ny = (long long)g.Nx * (g.Ny + 1);
-  if (t >= nx + ny) return;
-  const double dt = sc ? sc->dt : dt_arg;
-  const long long pl = g.plane;
-  const double mmE[3] = {1.0, 1.0 / 3.0, 4.0 / 45.0};
-  const double EzP[3] = {1.0, 1.0, 2.0 / 3.0};    // EzR = EzU
-  const double EzM[3] = {1.0, -1.0, 2.0 / 3.0};   // EzL = EzD
-  double d[3];
-  long long o;
-  int base;
-  if (t < nx) {
-    const int i = (int)(t % (g.Nx + 1)), j = 1 + (int)(t / (g.Nx + 1));
-    o = cix(g, i, j);
-    base = 0;
-    const double iy1 = 1.0 / g.hy1;
-    double a[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-      const double e = 0.5 * wgt(q) * (double)EZX[q * pl + o];
-      a[1] += e * iy1;
-      a[2] += e * (2 * lam(q) * iy1);
-    }
-    const double vU = EZV[cix(g, i, j)], vD = EZV[cix(g, i, j - 1)];
-#pragma unroll
-    for (int k = 0; k < 3; k++) d[k] = (a[k] - vU * EzP[k] / g.hy + vD * EzM[k] / g.hy) / mmE[k];
-  } else {
-    const long long u = t - nx;
-    const int i = 1 + (int)(u % g.Nx), j = (int)(u / g.Nx);
-    o = cix(g, i, j);
-    base = 3;
-    const double ix1 = 1.0 / g.hx1;
-    double a[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-      const double e = 0.5 * wgt(q) * (double)EZY[q * pl + o];
-      a[1] -= e * ix1;
-      a[2] -= e * (2 * lam(q) * ix1);
-    }
-    const double vR = EZV[cix(g, i, j)], vL = EZV[cix(g, i - 1, j)];
-#pragma unroll
-    for (int k = 0; k < 3; k++) d[k] = (a[k] + vR * EzP[k] / g.hx - vL * EzM[k] / g.hx) / mmE[k];
-  }
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    const long long oo = (base + k) * pl + o;
-    double r;
-    if (MODE == 0) r = d[k];
-    else if (MODE == 1) r = F[oo] + dt * d[k];
-    else if (MODE == 2) r = (3.0 / 4.0) * F0[oo] + (1.0 / 4.0) * (A[oo] + dt * d[k]);
-    else r = (1.0 / 3.0) * F0[oo] + (2.0 / 3.0) * (A[oo] + dt * d[k]);
-    out[oo] = r;
-  }
-}
-
-// whole-array combinations of RK3.f90 for the per-subroutine API
-// (mhd2d_rk_stage): out = a*X + b*(Y + dt*Z), elementwise
-__global__ __launch_bounds__(BS) void k_axpy(double* __restrict__ out, const double* __restrict__ X, double a,
-                                             const double* __restrict__ Y, double b, const double* __restrict__ Z,
-                                             double dt, long long n) {
-  const long long k = (long long)blockIdx.x * BS + threadIdx.x;
-  if (k >= n) return;
-  const double y = Y[k] + dt * Z[k];
-  out[k] = X ? a * X[k] + b * y : y;
-}
-
-// ---------------------------------------------------------------------------
-// positivity limiter of one cell (pp_Limiter.f90:10-81) on modes m[]
-__device__ __forceinline__ double glleval(const double* m, int n, int i1, int j1, int fam) {
-  const double x = fam == 0 ? lamL(i1) : lam(i1);
-  const double y = fam == 0 ? lam(j1) : lamL(j1);
-  double a = 0.0;
-#pragma unroll
-  for (int b = 0; b < 10; b++) {
-    const int p = PL(b, n);
-    if (p >= 0) a += m[p] * phi(b, x, y);
-  }
-  return a;
-}
-
-__device__ __forceinline__ void pp_cell(double m[NPL]) {
-  const double eps = 1e-13;
-  // density: eta = the LAST |(rbar - eps)/(rbar - rq)| below 1.  For
-  // doubles, RN(|a/b|) < 1 exactly when |a| < |b|, so only that point's
-  // quotient is formed.
-  double eta = 1.0;
-  const double rbar = m[PL(0, 0)];
-  const double na = rbar - eps;
-  double nb = 0.0;
-  bool any = false;
-#pragma unroll
-  for (int i1 = 0; i1 < 5; i1++)
-#pragma unroll
-    for (int j1 = 0; j1 < 5; j1++)
-#pragma unroll
-      for (int f = 0; f < 2; f++) {
-        const double b = rbar - glleval(m, 0, i1, j1, f);
-        if (fabs(na) < fabs(b)) { nb = b; any = true; }
-      }
-  if (any) eta = fabs(na / nb);
-  if (eta < 1.0) {
-#pragma unroll
-    for (int b = 1; b < 6; b++) m[PL(b, 0)] = 0.5 * eta * m[PL(b, 0)];
-  }
-  // pressure.  The reference keeps eta1 from the last negative-pressure
-  // point and takes eta = min(eta, eta1) at every point, i.e. eta is the
-  // minimum of eta1 over the negative-pressure points.  With the nodes the
-  // reference holds (-1, 0, 0, 0, 1) only 30 of its 50 points are distinct;
-  // their pressures are evaluated unrolled and the bisection
-  // (pp_Limiter.f90:101-125, at most 47 halvings) runs only where p < 0.
-  double ubar[8];
-#pragma unroll
-  for (int n = 0; n < 8; n++) ubar[n] = m[PL(0, n)];
-  unsigned neg = 0u;   // bit k: distinct point k has negative pressure
-#pragma unroll
-  for (int k = 0; k < 30; k++) {
-    // k < 15: family 1 (x = -1, 0, 1; y = Gauss); k >= 15: family 2
-    const int f = k / 15, a = (k % 15) / 5, b = k % 5;
-    const double x = f == 0 ? (a == 0 ? -1.0 : a == 1 ? 0.0 : 1.0) : lam(b);
-    const double y = f == 0 ? lam(b) : (a == 0 ? -1.0 : a == 1 ? 0.0 : 1.0);
-    double uq[8];
-#pragma unroll
-    for (int n = 0; n < 8; n++) {
-      double acc = 0.0;
-#pragma unroll
-      for (int bb = 0; bb < 10; bb++) {
-        const int p = PL(bb, n);
-        if (p >= 0) acc += m[p] * phi(bb, x, y);
-      }
-      uq[n] = acc;
-    }
-    if (pressure<double>(uq) < 0.0) neg |= 1u << k;
-  }
-  eta = 1.0;
-  while (neg) {
-    const int k = __builtin_ctz(neg);
-    neg &= neg - 1u;
-    const int f = k / 15, a = (k % 15) / 5, b = k % 5;
-    const double g = a == 0 ? -1.0 : a == 1 ? 0.0 : 1.0;
-    const double lb = b == 0 ? lam(0) : b == 1 ? lam(1) : b == 2 ? lam(2) : b == 3 ? lam(3) : lam(4);
-    const double x = f == 0 ? g : lb, y = f == 0 ? lb : g;
-    double uq[8];
-    {
-      MHD_NOCONTRACT
-      const double pv[10] = {1.0, x, y, x * x - 1.0 / 3.0, x * y, y * y - 1.0 / 3.0, x * x * x - 3.0 * x / 5.0,
-                             (x * x - 1.0 / 3.0) * (y), (x) * (y * y - 1.0 / 3.0), y * y * y - 3.0 * y / 5.0};
-#pragma unroll
-      for (int n = 0; n < 8; n++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int bb = 0; bb < 10; bb++) {
-          const int p = PL(bb, n);
-          if (p >= 0) acc = fma(m[p], pv[bb], acc);
-        }
-        uq[n] = acc;
-      }
-    }
-    double ta = 0.0, tb = 1.0, ut[8];
-    while (tb - ta > 1e-14) {
-      const double tq = 0.5 * (ta + tb);
-#pragma unroll
-      for (int n = 0; n < 8; n++) ut[n] = tq * uq[n] + (1 - tq) * ubar[n];
-      if (pressure<double>(ut) < 1e-12) tb = tq;
-      else ta = tq;
-    }
-    const double tq = ta;
-    double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-    for (int n = 0; n < 8; n++) {
-      const double sq = tq * uq[n] + (1 - tq) * ubar[n];
-      s1 += (sq - ubar[n]) * (sq - ubar[n]);
-      s2 += (uq[n] - ubar[n]) * (uq[n] - ubar[n]);
-    }
-    const double eta1 = sqrt(s1) / sqrt(s2);
-    if (eta1 < eta) eta = eta1;
-  }
-  if (eta < 1.0) eta = 0.5 * eta;
-#pragma unroll
-  for (int n = 0; n < 8; n++)
-#pragma unroll
-    for (int b = 1; b < 10; b++) {
-      const int p = PL(b, n);
-      if (p >= 0) m[p] = eta * m[p];
-    }
-}
-
-// div_free_Balsara.f90:15-78: in-cell B from the four faces
-__device__ __forceinline__ void divfree_cell(double m[NPL], const double* __restrict__ F, const Geo& g, int i, int j) {
-  const long long pl = g.plane;
-  const long long cR = cix(g, i, j), cL = cix(g, i - 1, j), cD = cix(g, i, j - 1);
-  const double a0R = F[0 * pl + cR], a1R = F[1 * pl + cR], a2R = F[2 * pl + cR];
-  const double a0L = F[0 * pl + cL], a1L = F[1 * pl + cL], a2L = F[2 * pl + cL];
-  const double b0U = F[3 * pl + cR], b1U = F[4 * pl + cR], b2U = F[5 * pl + cR];
-  const double b0D = F[3 * pl + cD], b1D = F[4 * pl + cD], b2D = F[5 * pl + cD];
-  const double rxy = g.hx / g.hy, ryx = g.hy / g.hx;
-  m[PL(0, 5)] = 0.5 * (a0R + a0L) + (1.0 / 6.0) * rxy * (b1U - b1D);
-  m[PL(1, 5)] = 0.5 * (a0R - a0L) + (1.0 / 15.0) * rxy * (b2U - b2D);
-  m[PL(2, 5)] = 0.5 * (a1R + a1L);
-  m[PL(3, 5)] = -(1.0 / 4.0) * rxy * (b1U - b1D);
-  m[PL(4, 5)] = 0.5 * (a1R - a1L);
-  m[PL(5, 5)] = 0.5 * (a2R + a2L);
-  m[PL(6, 5)] = -(1.0 / 6.0) * rxy * (b2U - b2D);
-  m[PL(8, 5)] = 0.5 * (a2R - a2L);
-  m[PL(0, 6)] = 0.5 * (b0U + b0D) + (1.0 / 6.0) * ryx * (a1R - a1L);
-  m[PL(2, 6)] = 0.5 * (b0U - b0D) + (1.0 / 15.0) * ryx * (a2R - a2L);
-  m[PL(1, 6)] = 0.5 * (b1U + b1D);
-  m[PL(5, 6)] = -(1.0 / 4.0) * ryx * (a1R - a1L);
-  m[PL(4, 6)] = 0.5 * (b1U - b1D);
-  m[PL(3, 6)] = 0.5 * (b2U + b2D);
-  m[PL(9, 6)] = -(1.0 / 6.0) * ryx * (a2R - a2L);
-  m[PL(7, 6)] = 0.5 * (b2U - b2D);
-}
-
-__device__ __forceinline__ bool is_bplane(int p) { return (p >= 30 && p < 42) || p >= 48; }
-
-// limiter pass (TVB_Limiter_P2.f90:16-121 + pp + div_free_Balsara) over
-// the whole array: interior cells are limited, halo cells copied (or, with
-// DIV, their B planes zeroed as div_free_Balsara.f90:13 does).
-// flags: indicator == 2/3 reads trouble[] (1 = limit).
-template <bool TVB, bool PP, bool DIV>
-__global__ __launch_bounds__(BS) void k_limit(const double* __restrict__ in, double* __restrict__ out,
-                                              const double* __restrict__ F, const int* __restrict__ trouble,
-                                              Geo g, double M, const DevScal* __restrict__ sc) {
-  const long long t = xcd_block() * BS + threadIdx.x;
-  const int W = g.Nx + 2;
-  if (t >= (long long)W * (g.Ny + 2)) return;
-  const int i = (int)(t % W), j = (int)(t / W);
-  const long long c = cix(g, i, j), pl = g.plane;
-  const bool interior = i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny;
-  if (!interior) {
-#pragma unroll 4
-    for (int p = 0; p < NPL; p++) out[p * pl + c] = (DIV && is_bplane(p)) ? 0.0 : in[p * pl + c];
-    return;
-  }
-  double m[NPL];
-#pragma unroll
-  for (int p = 0; p < NPL; p++) m[p] = in[p * pl + c];
-  if (TVB && (!trouble || trouble[(long long)(j - 1) * g.Nx + (i - 1)] == 1)) {
-    double U[8], dR[8], dL[8], r1[8], l1[8], Rm[8], Lm[8], Um[8], Dm[8];
-    int change = 0;
-    const long long cE = cix(g, i + 1, j), cW = cix(g, i - 1, j), cN = cix(g, i, j + 1), cS = cix(g, i, j - 1);
-#pragma unroll
-    for (int n = 0; n < 8; n++) {
-      U[n] = m[PL(0, n)];
-      dR[n] = in[PL(0, n) * pl + cE] - U[n];
-      dL[n] = U[n] - in[PL(0, n) * pl + cW];
-      r1[n] = m[PL(1, n)] + (2.0 / 3.0) * m[PL(3, n)];
-      l1[n] = m[PL(1, n)] - (2.0 / 3.0) * m[PL(3, n)];
-    }
-    tvb_dir<1>(U, dR, dL, r1, l1, M * (g.hx * g.hx), Rm, Lm);
-#pragma unroll
-    for (int n = 0; n < 8; n++)
-      if (fabs(Rm[n] - r1[n]) > 1e-6 || fabs(Lm[n] - l1[n]) > 1e-6) change |= 1 << n;
-#pragma unroll
-    for (int n = 0; n < 8; n++) {
-      dR[n] = in[PL(0, n) * pl + cN] - U[n];
-      dL[n] = U[n] - in[PL(0, n) * pl + cS];
-      r1[n] = m[PL(2, n)] + (2.0 / 3.0) * m[PL(5, n)];
-      l1[n] = m[PL(2, n)] - (2.0 / 3.0) * m[PL(5, n)];
-    }
-    tvb_dir<2>(U, dR, dL, r1, l1, M * (g.hy * g.hy), Um, Dm);
-#pragma unroll
-    for (int n = 0; n < 8; n++)
-      if (fabs(Um[n] - r1[n]) > 1e-6 || fabs(Dm[n] - l1[n]) > 1e-6) change |= 1 << n;
-#pragma unroll
-    for (int n = 0; n < 8; n++) {
-      if (!(change & (1 << n))) continue;
-#pragma unroll
-      for (int b = 6; b < 10; b++) {
-        const int p = PL(b, n);
-        if (p >= 0) m[p] = 0.0;
-      }
-      m[PL(4, n)] = 0.0;
-      m[PL(1, n)] = 0.5 * (Rm[n] + Lm[n]);
-      m[PL(2, n)] = 0.5 * (Um[n] + Dm[n]);
-      m[PL(3, n)] = (3.0 / 4.0) * (Rm[n] - Lm[n]);
-      m[PL(5, n)] = (3.0 / 4.0) * (Um[n] - Dm[n]);
-    }
-  }
-  if (PP) pp_cell(m);
-  if (DIV) divfree_cell(m, F, g, i, j);
-#pragma unroll
-  for (int p = 0; p < NPL; p++) out[p * pl + c] = m[p];
-}
 
 // set_bc.f90:5-27 over the halo ring: periodic (1) or outflow (2); the
 // x-then-y order of the reference makes the corners (map_x(i), map_y(j)).
@@ -791,79 +204,6 @@ __global__ void k_div_final(DevScal* sc, const double* partial, long long nblock
   }
 }
 
-// KXRCF_Detector.f90:180-231 (uG per cell, flags reset each call, Ck given)
-template <int SIDE>
-__device__ __forceinline__ double face_jump_int(const double* __restrict__ U, long long pl, long long cself,
-                                                long long cnb, int n) {
-  // sum_q w_q (u_self(q) - u_nb(q)) on the shared face
-  constexpr int OPP = SIDE == 0 ? 1 : SIDE == 1 ? 0 : SIDE == 2 ? 3 : 2;
-  double s = 0.0;
-#pragma unroll
-  for (int q = 0; q < 5; q++) {
-    const double x = lam(q), p2 = P2(x), p3 = P3(x), t23 = T23(x);
-    double a = 0.0, b = 0.0;
-#pragma unroll
-    for (int bb = 0; bb < 10; bb++) {
-      const int p = PL(bb, n);
-      if (p < 0) continue;
-      a += U[p * pl + cself] * phiTr(SIDE, bb, x, p2, p3, t23);
-      b += U[p * pl + cnb] * phiTr(OPP, bb, x, p2, p3, t23);
-    }
-    s += wgt(q) * (a - b);
-  }
-  return s;
-}
-
-__global__ __launch_bounds__(BS) void k_kxrcf(const double* __restrict__ U, int* __restrict__ trouble, Geo g,
-                                              double Ck, DevScal* sc) {
-  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
-  if (t >= (long long)g.Nx * g.Ny) return;
-  const int i = 1 + (int)(t % g.Nx), j = 1 + (int)(t / g.Nx);
-  const long long pl = g.plane, c = cix(g, i, j);
-  const long long cE = cix(g, i + 1, j), cW = cix(g, i - 1, j), cN = cix(g, i, j + 1), cS = cix(g, i, j - 1);
-  const bool inE = U[PL(0, 1) * pl + cE] < 0.0, inW = U[PL(0, 1) * pl + cW] > 0.0;
-  const bool inN = U[PL(0, 2) * pl + cN] < 0.0, inS = U[PL(0, 2) * pl + cS] > 0.0;
-  double Lij = 1e-12;
-  if (inE) Lij += 5 * g.hy;
-  if (inW) Lij += 5 * g.hy;
-  if (inN) Lij += 5 * g.hx;
-  if (inS) Lij += 5 * g.hx;
-  const double hk = pow(g.hx * g.hy, 0.25 * (2 + 1));
-  int flag = 0;
-  for (int n = 0; n < 8; n++) {
-    double I = 0.0;
-    if (inE) I += g.hy1 * face_jump_int<0>(U, pl, c, cE, n);
-    if (inW) I += g.hy1 * face_jump_int<1>(U, pl, c, cW, n);
-    if (inN) I += g.hx1 * face_jump_int<2>(U, pl, c, cN, n);
-    if (inS) I += g.hx1 * face_jump_int<3>(U, pl, c, cS, n);
-    double uinf = 0.0;
-#pragma unroll
-    for (int i1 = 0; i1 < 5; i1++)
-#pragma unroll
-      for (int j1 = 0; j1 < 5; j1++) {
-        double a = 0.0;
-#pragma unroll
-        for (int b = 0; b < 10; b++) {
-          const int p = PL(b, n);
-          if (p >= 0) a += U[p * pl + c] * phi(b, lam(i1), lam(j1));
-        }
-        uinf = fmax(uinf, fabs(a) + 1e-12);
-      }
-    if (fabs(I) / (hk * Lij * uinf) >= Ck) flag = 1;
-  }
-  trouble[t] = flag;
-  if (sc && flag) atomicAdd((unsigned long long*)&sc->nflag, 1ull);
-}
-
-// ---------------------------------------------------------------------------
-// initial conditions (init_data.f90:21-121; init1/3/4.txt; Rotor_functions.f90)
-struct ICPar {
-  int ic;
-  int pad;
-  double par[32];
-  double kx[16], ky[16], phu[16], phv[16];
-};
-
 __device__ void ic_eval(const ICPar& c, double x, double y, double U[8]) {
   const double pi = 4 * atan(1.0);
   double rho = 1, v1 = 0, v2 = 0, v3 = 0, B1 = 0, B2 = 0, B3 = 0, p = 1;
@@ -987,237 +327,13 @@ __global__ __launch_bounds__(BS) void k_init_faces(double* __restrict__ F, Geo g
 
 }  // namespace
 
-// ===========================================================================
-// host side
-
-struct mhd2d_ctx {
-  mhd2d_config cfg;
-  Geo g;
-  hipStream_t stream;
-  int device;
-  // state buffers (52 planes each); roles rotate inside mhd2d_step
-  double *S, *U0, *UI, *UII, *T;
-  double* du;
-  // face buffers (6 planes each)
-  double *F, *F0, *FI, *FII, *Ft, *dEz;
-  // face fluxes and EMFs
-  double *FX, *FY, *EZX, *EZY, *EZV;
-  int* trouble;
-  DevScal* sc;
-  double* partial;
-  ICPar* icp;
-  long long nblk_cells;
-  bool lh_done;
-  // live kernel timing (mhd2d_profile_*): event pairs on the context stream
-  bool prof;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_ev;
-  std::vector<hipEvent_t> ev_pool;
-  char err[512];
-};
-
 namespace {
-
-int fail(mhd2d_ctx* ctx, int code, const char* fmt, ...) {
-  if (ctx) {
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(ctx->err, sizeof ctx->err, fmt, ap);
-    va_end(ap);
-  }
-  return code;
-}
-
-#define HIPCHK(ctx, call)                                                              \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail(ctx, -2, "%s failed: %s", #call, hipGetErrorString(e_));             \
-  } while (0)
-#define LAUNCHCHK(ctx)                                                                 \
-  do {                                                                                 \
-    hipError_t e_ = hipGetLastError();                                                 \
-    if (e_ != hipSuccess) return fail(ctx, -3, "kernel launch: %s", hipGetErrorString(e_)); \
-  } while (0)
-
-inline unsigned nblk(long long n) { return (unsigned)((n + BS - 1) / BS); }
-
-// kernel classes for mhd2d_profile_get
-enum { KC_XFACE = 0, KC_YFACE, KC_VERTEX, KC_CELL, KC_CT, KC_LIMIT, KC_SETBC, KC_DT, KC_KXRCF, KC_DIAG, KC_N };
-
-hipEvent_t pool_event(mhd2d_ctx* ctx) {
-  if (!ctx->ev_pool.empty()) {
-    hipEvent_t e = ctx->ev_pool.back();
-    ctx->ev_pool.pop_back();
-    return e;
-  }
-  hipEvent_t e = nullptr;
-  (void)hipEventCreate(&e);
-  return e;
-}
-
-// brackets one launch with events when profiling is on
-struct Timed {
-  mhd2d_ctx* ctx;
-  int kc;
-  hipEvent_t a = nullptr, b = nullptr;
-  Timed(mhd2d_ctx* c, int k) : ctx(c), kc(k) {
-    if (ctx->prof) {
-      a = pool_event(ctx);
-      b = pool_event(ctx);
-      (void)hipEventRecord(a, ctx->stream);
-    }
-  }
-  ~Timed() {
-    if (ctx->prof && a && b) {
-      (void)hipEventRecord(b, ctx->stream);
-      ctx->prof_ev.push_back({kc, {a, b}});
-    }
-  }
-};
-
-template <typename F>
-int dispatch_ft(int ft, F&& f) {
-  switch (ft) {
-    case 1: return f(std::integral_constant<int, 1>{});
-    case 2: return f(std::integral_constant<int, 2>{});
-    case 3: return f(std::integral_constant<int, 3>{});
-    default: return f(std::integral_constant<int, 4>{});
-  }
-}
-
-// face fluxes + face EMFs (+ vertex EMFs when with_vertex); the face
-// quadrature runs in double (plain form) or float (mixed form)
-int launch_fluxes(mhd2d_ctx* ctx, const double* U, bool with_vertex, const DevScal* sc) {
-  const Geo& g = ctx->g;
-  const bool mixed = ctx->cfg.precision == MHD2D_MIXED;
-  return dispatch_ft(ctx->cfg.flux_type, [&](auto FTc) -> int {
-    constexpr int FT = decltype(FTc)::value;
-    const long long nxf = (long long)(g.Nx + 1) * g.Ny * 5, nyf = (long long)g.Nx * (g.Ny + 1) * 5;
-    {
-      Timed tm(ctx, KC_XFACE);
-      if (mixed)
-        hipLaunchKernelGGL((k_xface<FT, float>), dim3(nblk(nxf)), dim3(BS), 0, ctx->stream, U, (float*)ctx->FX,
-                           (float*)ctx->EZX, g, sc);
-      else
-        hipLaunchKernelGGL((k_xface<FT, double>), dim3(nblk(nxf)), dim3(BS), 0, ctx->stream, U, ctx->FX, ctx->EZX, g,
-                           sc);
-    }
-    {
-      Timed tm(ctx, KC_YFACE);
-      if (mixed)
-        hipLaunchKernelGGL((k_yface<FT, float>), dim3(nblk(nyf)), dim3(BS), 0, ctx->stream, U, (float*)ctx->FY,
-                           (float*)ctx->EZY, g, sc);
-      else
-        hipLaunchKernelGGL((k_yface<FT, double>), dim3(nblk(nyf)), dim3(BS), 0, ctx->stream, U, ctx->FY, ctx->EZY, g,
-                           sc);
-    }
-    if (with_vertex) {
-      Timed tm(ctx, KC_VERTEX);
-      const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
-      hipLaunchKernelGGL(k_vertex<FT>, dim3(nblk(nv)), dim3(BS), 0, ctx->stream, U, ctx->EZV, g, sc);
-    }
-    LAUNCHCHK(ctx);
-    return 0;
-  });
-}
-
-template <int MODE>
-void launch_cell_t(mhd2d_ctx* ctx, const double* S, const double* A, const double* U0, double* out,
-                   const DevScal* sc, double dt) {
-  const Geo& g = ctx->g;
-  const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
-  if (ctx->cfg.precision == MHD2D_MIXED)
-    hipLaunchKernelGGL((k_cell<MODE, float>), dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, (const float*)ctx->FX,
-                       (const float*)ctx->FY, A, U0, out, g, sc, dt);
-  else
-    hipLaunchKernelGGL((k_cell<MODE, double>), dim3(nblk(n)), dim3(BS), 0, ctx->stream, S, (const double*)ctx->FX,
-                       (const double*)ctx->FY, A, U0, out, g, sc, dt);
-}
-
-int launch_cell(mhd2d_ctx* ctx, int mode, const double* S, const double* A, const double* U0, double* out,
-                const DevScal* sc, double dt) {
-  Timed tm(ctx, KC_CELL);
-  switch (mode) {
-    case 0: launch_cell_t<0>(ctx, S, A, U0, out, sc, dt); break;
-    case 1: launch_cell_t<1>(ctx, S, A, U0, out, sc, dt); break;
-    case 2: launch_cell_t<2>(ctx, S, A, U0, out, sc, dt); break;
-    default: launch_cell_t<3>(ctx, S, A, U0, out, sc, dt); break;
-  }
-  LAUNCHCHK(ctx);
-  return 0;
-}
-
-template <int MODE>
-void launch_ct_t(mhd2d_ctx* ctx, const double* F, const double* A, const double* F0, double* out,
-                 const DevScal* sc, double dt) {
-  const Geo& g = ctx->g;
-  const long long n = (long long)(g.Nx + 1) * g.Ny + (long long)g.Nx * (g.Ny + 1);
-  if (ctx->cfg.precision == MHD2D_MIXED)
-    hipLaunchKernelGGL((k_ct<MODE, float>), dim3(nblk(n)), dim3(BS), 0, ctx->stream, (const float*)ctx->EZX,
-                       (const float*)ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt);
-  else
-    hipLaunchKernelGGL((k_ct<MODE, double>), dim3(nblk(n)), dim3(BS), 0, ctx->stream, (const double*)ctx->EZX,
-                       (const double*)ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt);
-}
-
-int launch_ct(mhd2d_ctx* ctx, int mode, const double* F, const double* A, const double* F0, double* out,
-              const DevScal* sc, double dt) {
-  Timed tm(ctx, KC_CT);
-  switch (mode) {
-    case 0: launch_ct_t<0>(ctx, F, A, F0, out, sc, dt); break;
-    case 1: launch_ct_t<1>(ctx, F, A, F0, out, sc, dt); break;
-    case 2: launch_ct_t<2>(ctx, F, A, F0, out, sc, dt); break;
-    default: launch_ct_t<3>(ctx, F, A, F0, out, sc, dt); break;
-  }
-  LAUNCHCHK(ctx);
-  return 0;
-}
 
 int launch_setbc(mhd2d_ctx* ctx, double* U, const DevScal* sc) {
   const Geo& g = ctx->g;
   const long long ring = 2LL * (g.Nx + 2) + 2LL * g.Ny;
   Timed tm(ctx, KC_SETBC);
   hipLaunchKernelGGL(k_setbc, dim3(nblk(ring * NPL)), dim3(BS), 0, ctx->stream, U, g, NPL, sc);
-  LAUNCHCHK(ctx);
-  return 0;
-}
-
-int launch_kxrcf(mhd2d_ctx* ctx, const double* U, DevScal* sc) {
-  const Geo& g = ctx->g;
-  Timed tm(ctx, KC_KXRCF);
-  hipLaunchKernelGGL(k_kxrcf, dim3(nblk((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream, U, ctx->trouble, g,
-                     ctx->cfg.Ck, sc);
-  LAUNCHCHK(ctx);
-  return 0;
-}
-
-// limiter pass in -> out (TVB with the configured pp / indicator, then
-// optionally div_free); trouble flags are computed first when needed
-int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool tvb, bool pp, bool div,
-                 DevScal* sc) {
-  const Geo& g = ctx->g;
-  const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
-  const int ind = ctx->cfg.indicator;
-  const int* trouble = nullptr;
-  if (tvb && ind == MHD2D_IND_NONE) tvb = false;
-  if (tvb && (ind == MHD2D_IND_KXRCF || ind == MHD2D_IND_MLP)) {
-    int rc = launch_kxrcf(ctx, in, sc);
-    if (rc) return rc;
-    trouble = ctx->trouble;
-  }
-  const double M = ctx->cfg.M;
-  const dim3 gr(nblk(n)), bl(BS);
-  Timed tm(ctx, KC_LIMIT);
-#define LIM(a, b, c) hipLaunchKernelGGL((k_limit<a, b, c>), gr, bl, 0, ctx->stream, in, out, F, trouble, g, M, sc)
-  if (tvb && pp && div) LIM(true, true, true);
-  else if (tvb && pp) LIM(true, true, false);
-  else if (tvb && div) LIM(true, false, true);
-  else if (tvb) LIM(true, false, false);
-  else if (pp && div) LIM(false, true, true);
-  else if (pp) LIM(false, true, false);
-  else if (div) LIM(false, false, true);
-  else LIM(false, false, false);
-#undef LIM
   LAUNCHCHK(ctx);
   return 0;
 }
@@ -1277,6 +393,9 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
       {(void**)&ctx->FX, xbytes},  {(void**)&ctx->FY, xbytes},  {(void**)&ctx->EZX, ebytes},
       {(void**)&ctx->EZY, ebytes}, {(void**)&ctx->EZV, (size_t)g.plane * sizeof(double)},
       {(void**)&ctx->trouble, (size_t)g.Nx * g.Ny * sizeof(int)},
+      {(void**)&ctx->tmask, (size_t)g.Nx * g.Ny * sizeof(int)},
+      {(void**)&ctx->tlist, (size_t)g.Nx * g.Ny * sizeof(int)},
+      {(void**)&ctx->trec, (size_t)32 * g.Nx * g.Ny * sizeof(double)},
       {(void**)&ctx->sc, sizeof(DevScal)},
       {(void**)&ctx->partial, (size_t)ctx->nblk_cells * sizeof(double)},
       {(void**)&ctx->icp, sizeof(ICPar)},
@@ -1313,8 +432,8 @@ int mhd2d_destroy(mhd2d_ctx* ctx) {
   ctx->prof_ev.~vector();
   ctx->ev_pool.~vector();
   void* ps[] = {ctx->S, ctx->U0, ctx->UI, ctx->UII, ctx->T, ctx->du, ctx->F, ctx->F0, ctx->FI, ctx->FII,
-                ctx->Ft, ctx->dEz, ctx->FX, ctx->FY, ctx->EZX, ctx->EZY, ctx->EZV, ctx->trouble, ctx->sc,
-                ctx->partial, ctx->icp};
+                ctx->Ft, ctx->dEz, ctx->FX, ctx->FY, ctx->EZX, ctx->EZY, ctx->EZV, ctx->trouble, ctx->tmask,
+                ctx->tlist, ctx->trec, ctx->sc, ctx->partial, ctx->icp};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1485,15 +604,7 @@ int mhd2d_lhez(mhd2d_ctx* ctx) {
   int rc = ctx_check(ctx);
   if (rc) return rc;
   if (!ctx->lh_done) return fail(ctx, -5, "lhez: requires a preceding lh on the same state (LhEz.f90 reads Lh's traces)");
-  const Geo& g = ctx->g;
-  rc = dispatch_ft(ctx->cfg.flux_type, [&](auto FTc) -> int {
-    constexpr int FT = decltype(FTc)::value;
-    const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
-    hipLaunchKernelGGL(k_vertex<FT>, dim3(nblk(nv)), dim3(BS), 0, ctx->stream, ctx->S, ctx->EZV, g, nullptr);
-    LAUNCHCHK(ctx);
-    return 0;
-  });
-  if (rc) return rc;
+  if ((rc = launch_vertex(ctx, ctx->S, nullptr))) return rc;
   return launch_ct(ctx, 0, nullptr, nullptr, nullptr, ctx->dEz, nullptr, 0.0);
 }
 
@@ -1505,7 +616,7 @@ int mhd2d_rk_stage(mhd2d_ctx* ctx, int32_t stage, double dt) {
   const size_t bs = ns * sizeof(double), bf = nf * sizeof(double);
   hipStream_t st = ctx->stream;
   auto ax = [&](double* out, const double* X, double a, const double* Y, double b, const double* Z, double d, long long n) {
-    hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(BS), 0, st, out, X, a, Y, b, Z, d, n);
+    (void)launch_axpy(ctx, out, X, a, Y, b, Z, d, n);
   };
   if (stage == 1) {
     ax(ctx->UI, nullptr, 0, ctx->S, 1, ctx->du, dt, ns);

@@ -1,0 +1,289 @@
+// mhd2d_cell.hip -- per-cell residual (volume and surface quadratures, mass
+// inverse) with the SSP-RK3 combination as its epilogue, the constrained-
+// transport update of the face B modes, and the elementwise stage update of
+// the per-subroutine API.
+#include "mhd2d_internal.hpp"
+
+namespace mhd2d_impl {
+namespace {
+
+// ---------------------------------------------------------------------------
+// cell residual (Lh.f90:11-68 volume, :268-304 surface + mass inverse) and
+// the RK3 combination of RK3.f90 as the epilogue:
+//   MODE 0: du                     (Lh alone)
+//   MODE 1: out = S + dt du        (uI,  RK3.f90:157)
+//   MODE 2: out = 3/4 U0 + 1/4 (UI + dt du)   (RK3.f90:183,187)
+//   MODE 3: out = 1/3 U0 + 2/3 (UII + dt du)  (RK3.f90:208-210)
+// The launch covers the halo ring too: there du = 0 (Lh.f90:5), so the
+// combination carries the start-of-step halos exactly as the reference's
+// whole-array statements do (the stale halos TVB later reads).
+//
+// T is the quadrature type (double: plain form; float: mixed form).  The
+// cell-average (mode 1) residual is accumulated in double in both forms.
+// The volume integral is sum-factorised row by row in eta: per row the
+// modes collapse to 1D coefficients in xi.  The cell's 52 modes are read
+// from HBM once into LDS (each lane owns a column of a [plane][lane] array,
+// so no barrier is needed) and re-read from LDS for every row; holding them
+// in registers next to the 52 accumulators would spill, and re-reading them
+// from global memory misses L2 at this occupancy (measured: 2.5x the
+// algorithmic traffic).  In the plain form the epilogue takes S from LDS too.
+template <int MODE, typename T, int CBS>
+__global__ __launch_bounds__(CBS) void k_cell(const double* __restrict__ S, const T* __restrict__ FX,
+                                              const T* __restrict__ FY, const double* __restrict__ A,
+                                              const double* __restrict__ U0, double* __restrict__ out, Geo g,
+                                              const DevScal* __restrict__ sc, double dt_arg) {
+  __shared__ T sm[NPL * CBS];
+  const int tid = threadIdx.x;
+  const long long t = xcd_block() * CBS + tid;
+  const int W = g.Nx + 2;
+  if (t >= (long long)W * (g.Ny + 2)) return;
+  const int i = (int)(t % W), j = (int)(t / W);
+  const long long c = cix(g, i, j);
+  const long long pl = g.plane;
+  const double dt = sc ? sc->dt : dt_arg;
+  const bool interior = i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny;
+  double acc0[8];       // mode-1 (cell average) residuals
+  T acc[NPL];           // higher modes (slots PL(0,n) unused)
+#pragma unroll
+  for (int n = 0; n < 8; n++) acc0[n] = 0.0;
+#pragma unroll
+  for (int p = 0; p < NPL; p++) acc[p] = T(0);
+  if (interior) {
+#pragma unroll
+    for (int p = 0; p < NPL; p++) sm[p * CBS + tid] = T(S[p * pl + c]);
+    const T ihx1 = T(1.0 / g.hx1), ihy1 = T(1.0 / g.hy1);
+#pragma unroll
+    for (int j1 = 0; j1 < 5; j1++) {
+      const double y = lam(j1);
+      const T ty = T(y), tP2y = T(P2(y)), tP3y = T(P3(y));
+      int off = tid;
+      asm volatile("" : "+v"(off));   // re-read the modes from LDS for every row
+      const T* m = sm + off;
+      // U(x, y_j) = c0 + c1 x + c2 P2(x) + c3 P3(x) per variable
+      T c0[8], c1[8], c2[8], c3b = T(0);
+#pragma unroll
+      for (int n = 0; n < 8; n++) {
+        const T m0 = m[PL(0, n) * CBS], m1 = m[PL(1, n) * CBS];
+        const T m2 = m[PL(2, n) * CBS], m3 = m[PL(3, n) * CBS];
+        const T m4 = m[PL(4, n) * CBS], m5 = m[PL(5, n) * CBS];
+        c0[n] = m0 + m2 * ty + m5 * tP2y;
+        c1[n] = m1 + m4 * ty;
+        c2[n] = m3;
+        if (n == 5) {   // Bx: phi7 = P3(xi), phi9 = xi P2(eta)
+          c1[n] += m[PL(8, 5) * CBS] * tP2y;
+          c3b = m[PL(6, 5) * CBS];
+        }
+        if (n == 6) {   // By: phi8 = P2(xi) eta, phi10 = P3(eta)
+          c2[n] += m[PL(7, 6) * CBS] * ty;
+          c0[n] += m[PL(9, 6) * CBS] * tP3y;
+        }
+      }
+#pragma unroll
+      for (int i1 = 0; i1 < 5; i1++) {
+        const double x = lam(i1);
+        const T tx = T(x), tP2x = T(P2(x)), tP3x = T(P3(x));
+        T Uq[8], Fx[8], Fy[8];
+#pragma unroll
+        for (int n = 0; n < 8; n++) Uq[n] = c0[n] + c1[n] * tx + c2[n] * tP2x;
+        Uq[5] += c3b * tP3x;
+        lh_flux_xy<T>(Uq, Fx, Fy);
+#pragma unroll
+        for (int n = 0; n < 8; n++) { Fx[n] *= ihx1; Fy[n] *= ihy1; }
+        const double ww = 0.25 * wgt(i1) * wgt(j1);
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+#pragma unroll
+          for (int b = 1; b < 10; b++) {
+            const int p = PL(b, n);
+            if (p < 0) continue;
+            const double cx = ww * dphix(b, x, y), cy = ww * dphiy(b, x, y);
+            if (cx != 0.0) acc[p] += T(cx) * Fx[n];
+            if (cy != 0.0) acc[p] += T(cy) * Fy[n];
+          }
+        }
+      }
+    }
+    // surface integrals
+    const T sx = T(0.5 / g.hx), sy = T(0.5 / g.hy);
+    const double sxd = 0.5 / g.hx, syd = 0.5 / g.hy;
+    const long long cL = cix(g, i - 1, j), cD = cix(g, i, j - 1);
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      const double s = lam(q), p2 = P2(s), p3 = P3(s), t23 = T23(s);
+      const double wq = wgt(q);
+#pragma unroll
+      for (int n = 0; n < 8; n++) {
+        const T fr = FX[(q * 8 + n) * pl + c], fl = FX[(q * 8 + n) * pl + cL];
+        const T fu = FY[(q * 8 + n) * pl + c], fd = FY[(q * 8 + n) * pl + cD];
+        acc0[n] -= sxd * wq * ((double)fr - (double)fl);
+        acc0[n] -= syd * wq * ((double)fu - (double)fd);
+#pragma unroll
+        for (int b = 1; b < 10; b++) {
+          const int p = PL(b, n);
+          if (p < 0) continue;
+          acc[p] -= sx * T(wq) * (fr * T(phiTr(0, b, s, p2, p3, t23)) - fl * T(phiTr(1, b, s, p2, p3, t23)));
+          acc[p] -= sy * T(wq) * (fu * T(phiTr(2, b, s, p2, p3, t23)) - fd * T(phiTr(3, b, s, p2, p3, t23)));
+        }
+      }
+    }
+  }
+  // epilogue: mass inverse + RK combination, in double
+#pragma unroll
+  for (int p = 0; p < NPL; p++) {
+    const long long o = p * pl + c;
+    const bool avg = p < 48 && (p % 6) == 0;
+    int b = p < 48 ? p % 6 : (p == 48 ? 6 : p == 49 ? 8 : p == 50 ? 7 : 9);
+    const double du = avg ? acc0[p / 6] : (double)acc[p] * (1.0 / mmass(b));
+    double r;
+    if (MODE == 0) r = du;
+    else if (MODE == 1) r = (sizeof(T) == 8 && interior ? (double)sm[p * CBS + tid] : S[o]) + dt * du;
+    else if (MODE == 2) r = (3.0 / 4.0) * U0[o] + (1.0 / 4.0) * (A[o] + dt * du);
+    else r = (1.0 / 3.0) * U0[o] + (2.0 / 3.0) * (A[o] + dt * du);
+    out[o] = r;
+  }
+}
+
+// constrained transport: LhEz.f90:71-96 (+ the face combinations of
+// RK3.f90:159-212 for MODE 1..3 as in k_cell).  Face planes: 0..2 Bx modes
+// on x faces (i = 0..Nx, j = 1..Ny), 3..5 By modes on y faces (i = 1..Nx,
+// j = 0..Ny).
+template <int MODE, typename T>
+__global__ __launch_bounds__(BS) void k_ct(const T* __restrict__ EZX, const T* __restrict__ EZY,
+                                           const double* __restrict__ EZV, const double* __restrict__ F,
+                                           const double* __restrict__ A, const double* __restrict__ F0,
+                                           double* __restrict__ out, Geo g, const DevScal* __restrict__ sc,
+                                           double dt_arg) {
+  const long long t = xcd_block() * BS + threadIdx.x;
+  const long long nx = (long long)(g.Nx + 1) * g.Ny, ny = (long long)g.Nx * (g.Ny + 1);
+  if (t >= nx + ny) return;
+  const double dt = sc ? sc->dt : dt_arg;
+  const long long pl = g.plane;
+  const double mmE[3] = {1.0, 1.0 / 3.0, 4.0 / 45.0};
+  const double EzP[3] = {1.0, 1.0, 2.0 / 3.0};    // EzR = EzU
+  const double EzM[3] = {1.0, -1.0, 2.0 / 3.0};   // EzL = EzD
+  double d[3];
+  long long o;
+  int base;
+  if (t < nx) {
+    const int i = (int)(t % (g.Nx + 1)), j = 1 + (int)(t / (g.Nx + 1));
+    o = cix(g, i, j);
+    base = 0;
+    const double iy1 = 1.0 / g.hy1;
+    double a[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      const double e = 0.5 * wgt(q) * (double)EZX[q * pl + o];
+      a[1] += e * iy1;
+      a[2] += e * (2 * lam(q) * iy1);
+    }
+    const double vU = EZV[cix(g, i, j)], vD = EZV[cix(g, i, j - 1)];
+#pragma unroll
+    for (int k = 0; k < 3; k++) d[k] = (a[k] - vU * EzP[k] / g.hy + vD * EzM[k] / g.hy) / mmE[k];
+  } else {
+    const long long u = t - nx;
+    const int i = 1 + (int)(u % g.Nx), j = (int)(u / g.Nx);
+    o = cix(g, i, j);
+    base = 3;
+    const double ix1 = 1.0 / g.hx1;
+    double a[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      const double e = 0.5 * wgt(q) * (double)EZY[q * pl + o];
+      a[1] -= e * ix1;
+      a[2] -= e * (2 * lam(q) * ix1);
+    }
+    const double vR = EZV[cix(g, i, j)], vL = EZV[cix(g, i - 1, j)];
+#pragma unroll
+    for (int k = 0; k < 3; k++) d[k] = (a[k] + vR * EzP[k] / g.hx - vL * EzM[k] / g.hx) / mmE[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const long long oo = (base + k) * pl + o;
+    double r;
+    if (MODE == 0) r = d[k];
+    else if (MODE == 1) r = F[oo] + dt * d[k];
+    else if (MODE == 2) r = (3.0 / 4.0) * F0[oo] + (1.0 / 4.0) * (A[oo] + dt * d[k]);
+    else r = (1.0 / 3.0) * F0[oo] + (2.0 / 3.0) * (A[oo] + dt * d[k]);
+    out[oo] = r;
+  }
+}
+
+// whole-array combinations of RK3.f90 for the per-subroutine API
+// (mhd2d_rk_stage): out = a*X + b*(Y + dt*Z), elementwise
+__global__ __launch_bounds__(BS) void k_axpy(double* __restrict__ out, const double* __restrict__ X, double a,
+                                             const double* __restrict__ Y, double b, const double* __restrict__ Z,
+                                             double dt, long long n) {
+  const long long k = (long long)blockIdx.x * BS + threadIdx.x;
+  if (k >= n) return;
+  const double y = Y[k] + dt * Z[k];
+  out[k] = X ? a * X[k] + b * y : y;
+}
+
+}  // namespace
+
+// block sizes of k_cell: its LDS copy of the state is 52 values per lane
+// (plain form: 128 lanes x 416 B = 53 KB per block, 3 blocks per CU)
+constexpr int CBS_F64 = 128, CBS_F32 = 256;
+
+template <int MODE>
+void launch_cell_t(mhd2d_ctx* ctx, const double* S, const double* A, const double* U0, double* out,
+                   const DevScal* sc, double dt) {
+  const Geo& g = ctx->g;
+  const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
+  if (ctx->cfg.precision == MHD2D_MIXED)
+    hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
+                       dim3(CBS_F32), 0, ctx->stream, S, (const float*)ctx->FX, (const float*)ctx->FY, A, U0, out, g,
+                       sc, dt);
+  else
+    hipLaunchKernelGGL((k_cell<MODE, double, CBS_F64>), dim3((unsigned)((n + CBS_F64 - 1) / CBS_F64)),
+                       dim3(CBS_F64), 0, ctx->stream, S, (const double*)ctx->FX, (const double*)ctx->FY, A, U0, out,
+                       g, sc, dt);
+}
+
+int launch_cell(mhd2d_ctx* ctx, int mode, const double* S, const double* A, const double* U0, double* out,
+                const DevScal* sc, double dt) {
+  Timed tm(ctx, KC_CELL);
+  switch (mode) {
+    case 0: launch_cell_t<0>(ctx, S, A, U0, out, sc, dt); break;
+    case 1: launch_cell_t<1>(ctx, S, A, U0, out, sc, dt); break;
+    case 2: launch_cell_t<2>(ctx, S, A, U0, out, sc, dt); break;
+    default: launch_cell_t<3>(ctx, S, A, U0, out, sc, dt); break;
+  }
+  LAUNCHCHK(ctx);
+  return 0;
+}
+
+template <int MODE>
+void launch_ct_t(mhd2d_ctx* ctx, const double* F, const double* A, const double* F0, double* out,
+                 const DevScal* sc, double dt) {
+  const Geo& g = ctx->g;
+  const long long n = (long long)(g.Nx + 1) * g.Ny + (long long)g.Nx * (g.Ny + 1);
+  if (ctx->cfg.precision == MHD2D_MIXED)
+    hipLaunchKernelGGL((k_ct<MODE, float>), dim3(nblk(n)), dim3(BS), 0, ctx->stream, (const float*)ctx->EZX,
+                       (const float*)ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt);
+  else
+    hipLaunchKernelGGL((k_ct<MODE, double>), dim3(nblk(n)), dim3(BS), 0, ctx->stream, (const double*)ctx->EZX,
+                       (const double*)ctx->EZY, ctx->EZV, F, A, F0, out, g, sc, dt);
+}
+
+int launch_ct(mhd2d_ctx* ctx, int mode, const double* F, const double* A, const double* F0, double* out,
+              const DevScal* sc, double dt) {
+  Timed tm(ctx, KC_CT);
+  switch (mode) {
+    case 0: launch_ct_t<0>(ctx, F, A, F0, out, sc, dt); break;
+    case 1: launch_ct_t<1>(ctx, F, A, F0, out, sc, dt); break;
+    case 2: launch_ct_t<2>(ctx, F, A, F0, out, sc, dt); break;
+    default: launch_ct_t<3>(ctx, F, A, F0, out, sc, dt); break;
+  }
+  LAUNCHCHK(ctx);
+  return 0;
+}
+
+int launch_axpy(mhd2d_ctx* ctx, double* out, const double* X, double a, const double* Y, double b,
+                const double* Z, double dt, long long n) {
+  hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(BS), 0, ctx->stream, out, X, a, Y, b, Z, dt, n);
+  LAUNCHCHK(ctx);
+  return 0;
+}
+
+}  // namespace mhd2d_impl

@@ -1,0 +1,204 @@
+// mhd2d_internal.hpp -- declarations shared by the translation units of
+// libmhd2d.so: device scalars, the context, launch helpers and the launch
+// functions each kernel file provides.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <utility>
+#include <vector>
+
+#include "../../include/mhd2d.h"
+#include "mhd2d_physics.hpp"
+#include "mhd2d_tables.hpp"
+#include "mhd2d_tvb.hpp"
+
+namespace mhd2d_impl {
+using namespace mhd;
+
+constexpr int BS = 256;
+
+// device scalars of the time loop (COMMON /time/ and the diagnostics)
+struct DevScal {
+  double t, dt, tend, CFL;
+  double alphax, alphay, umax, pmin, totaldiv;
+  unsigned long long ax_bits, ay_bits, umax_bits;
+  long long pmin_key;
+  long long count;
+  long long nflag;
+  unsigned int nlist;   // length of the troubled-cell list of the current limiter pass
+  unsigned int pad1;
+  int active;
+  int pad;
+};
+
+__device__ __forceinline__ long long dkey(double x) {   // order-preserving key of a double
+  long long b = __double_as_longlong(x);
+  return b >= 0 ? b : b ^ 0x7fffffffffffffffLL;
+}
+__device__ __forceinline__ double dunkey(long long k) {
+  return __longlong_as_double(k >= 0 ? k : k ^ 0x7fffffffffffffffLL);
+}
+
+// blocks of consecutive ids go to the same XCD (round-robin dispatch over 8)
+__device__ __forceinline__ long long xcd_block() {
+  const unsigned b = blockIdx.x, n = gridDim.x;
+  const unsigned full = n & ~7u;
+  if (b >= full) return b;
+  return (long long)(b & 7u) * (full >> 3) + (b >> 3);
+}
+
+// select one of five compile-time node values by a lane-varying index
+__device__ __forceinline__ double sel5(int q, double a0, double a1, double a2, double a3, double a4) {
+  double r = a0;
+  r = q == 1 ? a1 : r;
+  r = q == 2 ? a2 : r;
+  r = q == 3 ? a3 : r;
+  r = q == 4 ? a4 : r;
+  return r;
+}
+
+// trace of all 8 variables of one cell on one side at node q, in type T
+template <int SIDE, typename T>
+__device__ __forceinline__ void trace_at(const double* __restrict__ U, long long plane, long long c, int q,
+                                         T out[8]) {
+  constexpr double L0 = lam(0), L1 = lam(1), L2 = lam(2), L3 = lam(3), L4 = lam(4);
+  const double s = sel5(q, L0, L1, L2, L3, L4);
+  const double p2 = sel5(q, P2(L0), P2(L1), P2(L2), P2(L3), P2(L4));
+  const double p3 = sel5(q, P3(L0), P3(L1), P3(L2), P3(L3), P3(L4));
+  const double t23 = sel5(q, T23(L0), T23(L1), T23(L2), T23(L3), T23(L4));
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    T acc = T(0);
+#pragma unroll
+    for (int b = 0; b < 10; b++) {
+      const int p = PL(b, n);
+      if (p >= 0) acc += T(U[p * plane + c]) * T(phiTr(SIDE, b, s, p2, p3, t23));
+    }
+    out[n] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// initial conditions (init_data.f90:21-121; init1/3/4.txt; Rotor_functions.f90)
+struct ICPar {
+  int ic;
+  int pad;
+  double par[32];
+  double kx[16], ky[16], phu[16], phv[16];
+};
+
+// kernel classes for mhd2d_profile_get
+enum { KC_XFACE = 0, KC_YFACE, KC_VERTEX, KC_CELL, KC_CT, KC_LIMIT, KC_SETBC, KC_DT, KC_KXRCF, KC_DIAG, KC_N };
+
+}  // namespace mhd2d_impl
+
+// ===========================================================================
+// host side
+
+struct mhd2d_ctx {
+  mhd2d_config cfg;
+  mhd::Geo g;
+  hipStream_t stream;
+  int device;
+  // state buffers (52 planes each); roles rotate inside mhd2d_step
+  double *S, *U0, *UI, *UII, *T;
+  double* du;
+  // face buffers (6 planes each)
+  double *F, *F0, *FI, *FII, *Ft, *dEz;
+  // face fluxes and EMFs
+  double *FX, *FY, *EZX, *EZY, *EZV;
+  int* trouble;        // detector decision per interior cell (1 = TVB applied)
+  int* tmask;          // TVB change bits per interior cell (0 = unchanged)
+  int* tlist;          // interior-cell indices the detector flagged (length sc->nlist)
+  double* trec;        // modified edge values of changed cells: 32 planes over the interior cells
+  mhd2d_impl::DevScal* sc;
+  double* partial;
+  mhd2d_impl::ICPar* icp;
+  long long nblk_cells;
+  bool lh_done;
+  // live kernel timing (mhd2d_profile_*): event pairs on the context stream
+  bool prof;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_ev;
+  std::vector<hipEvent_t> ev_pool;
+  char err[512];
+};
+
+namespace mhd2d_impl {
+
+inline int fail(mhd2d_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(ctx->err, sizeof ctx->err, fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ctx, -2, "%s failed: %s", #call, hipGetErrorString(e_));             \
+  } while (0)
+#define LAUNCHCHK(ctx)                                                                 \
+  do {                                                                                 \
+    hipError_t e_ = hipGetLastError();                                                 \
+    if (e_ != hipSuccess) return fail(ctx, -3, "kernel launch: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+inline unsigned nblk(long long n) { return (unsigned)((n + BS - 1) / BS); }
+
+
+inline hipEvent_t pool_event(mhd2d_ctx* ctx) {
+  if (!ctx->ev_pool.empty()) {
+    hipEvent_t e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// brackets one launch with events when profiling is on
+struct Timed {
+  mhd2d_ctx* ctx;
+  int kc;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(mhd2d_ctx* c, int k) : ctx(c), kc(k) {
+    if (ctx->prof) {
+      a = pool_event(ctx);
+      b = pool_event(ctx);
+      (void)hipEventRecord(a, ctx->stream);
+    }
+  }
+  ~Timed() {
+    if (ctx->prof && a && b) {
+      (void)hipEventRecord(b, ctx->stream);
+      ctx->prof_ev.push_back({kc, {a, b}});
+    }
+  }
+};
+
+// launch functions (stream-ordered on ctx->stream; 0 or a negative status)
+int launch_fluxes(mhd2d_ctx* ctx, const double* U, bool with_vertex, const DevScal* sc);     // mhd2d_faces.hip
+int launch_vertex(mhd2d_ctx* ctx, const double* U, const DevScal* sc);                      // mhd2d_faces.hip
+int launch_cell(mhd2d_ctx* ctx, int mode, const double* S, const double* A, const double* U0, double* out,
+                const DevScal* sc, double dt);                                                // mhd2d_cell.hip
+int launch_ct(mhd2d_ctx* ctx, int mode, const double* F, const double* A, const double* F0, double* out,
+              const DevScal* sc, double dt);                                                  // mhd2d_cell.hip
+int launch_axpy(mhd2d_ctx* ctx, double* out, const double* X, double a, const double* Y, double b,
+                const double* Z, double dt, long long n);                                     // mhd2d_cell.hip
+int launch_kxrcf(mhd2d_ctx* ctx, const double* U, DevScal* sc);                               // mhd2d_limit.hip
+int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool tvb, bool pp, bool div,
+                 DevScal* sc);                                                                // mhd2d_limit.hip
+
+}  // namespace mhd2d_impl

@@ -16,8 +16,15 @@
 #   P7 HLL_Flux_2D.f90:8    EzRU..EzLD are read uninitialised at :176-177;
 #                           they are set to the corner fields v*B1 - u*B2
 #                           (calculate_Ez, as LF_Flux_2D.f90:18-21 does)
-# KXRCF_Detector.f90 does not compile as shipped (uG undeclared) and is not
-# built.  main.f90 is replaced by oracle/ref_driver.f90.
+#   P6 KXRCF_Detector.f90   does not compile as shipped (uG undeclared at
+#                           :17,91): uG is declared as a saved local of the
+#                           shape its loops index; Ck (:7, shipped 0, which
+#                           flags every cell) is set to the variant's value;
+#                           Is_Trouble_Cell is reset at entry (the shipped
+#                           code never clears a flag).  In variants *_kx the
+#                           commented KXRCF gating of TVB_Limiter_P2.f90:12,
+#                           22,116 is enabled.
+# main.f90 is replaced by oracle/ref_driver.f90.
 set -euo pipefail
 
 REF=${REF:-/root/reference/GDF_Balsara_P2_0316}
@@ -32,26 +39,29 @@ if [ ! -d "$REF" ]; then
 fi
 mkdir -p "$OUT"
 
-# name Nx Ny flux_type pp openmp
+# name Nx Ny flux_type pp openmp kxrcf-gating (Ck = 1 everywhere)
 VARIANTS=${VARIANTS:-"
-r16_f4 16 16 4 0 0
-r16_f2 16 16 2 0 0
-r16_f1 16 16 1 0 0
-r16_f3 16 16 3 0 0
-r16_f4_pp 16 16 4 1 0
-r12x8_f4 12 8 4 0 0
-r32_f4_pp 32 32 4 1 0
-r32_f2 32 32 2 0 0
-r256_f4_pp_omp 256 256 4 1 1
+r16_f4 16 16 4 0 0 0
+r16_f2 16 16 2 0 0 0
+r16_f1 16 16 1 0 0 0
+r16_f3 16 16 3 0 0 0
+r16_f4_pp 16 16 4 1 0 0
+r12x8_f4 12 8 4 0 0 0
+r32_f4_pp 32 32 4 1 0 0
+r32_f2 32 32 2 0 0 0
+r32_f4_pp_kx 32 32 4 1 0 1
+r32_f2_kx 32 32 2 0 0 1
+r256_f4_pp_omp 256 256 4 1 1 0
 "}
+CK=${CK:-1}
 
 SRCS="get_GLP get_basis set_bc Lh LhEz eigenvalueMm HLL_Flux LF_Flux HLLC_Flux HLLD_Flux
 HLL_Flux_2D LF_Flux_2D HLLC_Flux_2D HLLD_Flux_2D HLL1 MHD_flux calculate_Ez TVB_Limiter_P2
 minmod minmodB eigenmatrix pp_Limiter pressure div_free_Balsara calculate_dt calculate_umax
-calculate_pmin calculate_totaldiv RK3 Rotor_functions"
+calculate_pmin calculate_totaldiv RK3 Rotor_functions KXRCF_Detector"
 
 build_one() {
-  local name=$1 nx=$2 ny=$3 ft=$4 pp=$5 omp=$6
+  local name=$1 nx=$2 ny=$3 ft=$4 pp=$5 omp=$6 kx=$7
   local w
   w=$(mktemp -d /tmp/gdf_ref_XXXXXX)
   cp "$REF"/*.f90 "$REF"/*.txt "$w"/
@@ -67,6 +77,16 @@ build_one() {
   # P5
   if [ "$pp" = 1 ]; then
     sed -i "s/^\(\s*\)!call pp_Limiter/\1call pp_Limiter/" "$w/TVB_Limiter_P2.f90"
+  fi
+  # P6
+  sed -i -e "/^\s*real Iij(NumEq),Lij,uinfty(NumEq),Ck/a\\
+    real, save :: uG(Nx,Ny,NumGLP,NumGLP,NumEq)" \
+         -e "s/^\(\s*\)Ck = 0\s*$/\1Ck = $CK\\
+\1Is_Trouble_Cell = 0/" "$w/KXRCF_Detector.f90"
+  if [ "$kx" = 1 ]; then
+    sed -i -e "s/^\(\s*\)!call KXRCF_Detector/\1call KXRCF_Detector/" \
+           -e "s/^\(\s*\)!if (Is_Trouble_Cell(i,j) == 1) then/\1if (Is_Trouble_Cell(i,j) == 1) then/" \
+           -e "s/^\(\s*\)!end if\s*$/\1end if/" "$w/TVB_Limiter_P2.f90"
   fi
   # P7
   sed -i "/^\s*ByLD = ULD1(7)/a\\
@@ -91,7 +111,7 @@ build_one() {
   echo "built $OUT/$name"
 }
 
-echo "$VARIANTS" | while read -r name nx ny ft pp omp; do
+echo "$VARIANTS" | while read -r name nx ny ft pp omp kx; do
   [ -z "$name" ] && continue
-  build_one "$name" "$nx" "$ny" "$ft" "$pp" "$omp"
+  build_one "$name" "$nx" "$ny" "$ft" "$pp" "$omp" "${kx:-0}"
 done

@@ -221,6 +221,7 @@ def run_ref(variant, mode, uh, Bx, By, domain=(0.0, 2 * np.pi, 0.0, 2 * np.pi), 
     o["dEz1"] = take((Nx + 1, Ny, 3))
     o["dEz2"] = take((Nx, Ny + 1, 3))
     o["EzVertex"] = take((Nx + 1, Ny + 1))
+    o["trouble"] = take((Nx, Ny)).astype(np.int32)
     return o
 
 

@@ -9,8 +9,10 @@
 ! usage: ref_driver MODE INPUT OUTPUT
 !   INPUT : stream file = 16 doubles header + uh + Bx + By (reference layout)
 !           header: xa xb ya yb bcR bcL bcU bcD M tend t dt nsteps 0 0 0
-!   MODE  : setbc | lh | tvb | pp | divfree | dt | diag | run | bench | tables | eig
-!           | init3 | init1 | init4
+!   MODE  : setbc | lh | tvb | pp | divfree | kxrcf | dt | diag | run | bench
+!           | tables | eig | init3 | init1 | init4
+!   OUTPUT: 8 scalars, 8 mesh values, uh, Bx, By, du, dEz1, dEz2, EzVertex,
+!           Is_Trouble_Cell (as doubles)
 program ref_driver
     include 'com.txt'
     character(len=512) :: mode, fin, fout
@@ -65,6 +67,8 @@ program ref_driver
         call pp_Limiter
     case ('divfree')
         call div_free_Balsara
+    case ('kxrcf')
+        call KXRCF_Detector
     case ('dt')
         call calculate_dt
         scal(2) = dt
@@ -183,5 +187,6 @@ program ref_driver
     write(11) scal
     write(11) xa, xb, ya, yb, hx, hy, M, tend
     write(11) uh, Bx, By, du, dEz1, dEz2, EzVertex
+    write(11) dble(Is_Trouble_Cell)
     close(11)
 end program ref_driver

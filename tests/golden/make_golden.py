@@ -154,6 +154,17 @@ def main():
          dt=rd["dt"], alphax=rd["alphax"], alphay=rd["alphay"],
          umax=rg["umax"], totaldiv=rg["totaldiv"], pmin=rg["pmin"])
 
+    # --- KXRCF_Detector (P6: uG declared, flags reset, Ck = 1) -------------
+    r = O.run_ref("r16_f4", "kxrcf", su, sx, sy, domain=dom_ot)
+    print(f"  kxrcf stress 16: {int(r['trouble'].sum())} of 256 flagged")
+    save("kxrcf_16", uh=pack(su), Bx=sx, By=sy, domain=dom_ot, Ck=1.0, trouble=r["trouble"])
+    r0 = O.run_ref("r32_f4_pp_kx", "init4", *zeros(32, 32))
+    dom_r = np.array([r0["xa"], r0["xb"], r0["ya"], r0["yb"]])
+    r = O.run_ref("r32_f4_pp_kx", "kxrcf", periodic_halo(r0["uh"]), r0["Bx"], r0["By"], domain=dom_r)
+    print(f"  kxrcf rotor 32: {int(r['trouble'].sum())} of 1024 flagged")
+    save("kxrcf_rotor_32", uh=pack(periodic_halo(r0["uh"])), Bx=r0["Bx"], By=r0["By"], domain=dom_r, Ck=1.0,
+         trouble=r["trouble"])
+
     # --- whole RK3 runs (prologue + steps to tend) ---------------------------
     runs = [
         ("run_f4_16", "r16_f4", (16, 16), 1, 50.0, 0.05),
@@ -163,6 +174,8 @@ def main():
         ("run_f4_pp_16_outflow", "r16_f4_pp", (16, 16), 2, 0.0, 0.05),
         ("run_f4_pp_32", "r32_f4_pp", (32, 32), 1, 50.0, 0.05),
         ("run_f2_32", "r32_f2", (32, 32), 1, 10.0, 0.08),
+        # TVB gated by KXRCF (TVB_Limiter_P2.f90:12,22,116 enabled, Ck = 1)
+        ("run_f2_kx_32", "r32_f2_kx", (32, 32), 1, 50.0, 0.3),
     ]
     for name, variant, (Nx, Ny), bc, M, tend in runs:
         ui, bxi, byi = ot_init(variant, Nx, Ny)
@@ -180,6 +193,11 @@ def main():
     save("run_rotor_f4_pp_32", uh0=pack(r0["uh"]), Bx0=r0["Bx"], By0=r0["By"], domain=dom_r, bc=1, M=10.0,
          tend=0.01, uh=pack(r["uh"]), Bx=r["Bx"], By=r["By"], t=r["t"], count=r["count"],
          umax=r["umax"], totaldiv=r["totaldiv"], pmin=r["pmin"])
+    r = O.run_ref("r32_f4_pp_kx", "run", r0["uh"], r0["Bx"], r0["By"], domain=dom_r, M=10.0, tend=0.01)
+    print(f"  rotor, KXRCF-gated TVB: {int(r['count'])} steps, {int(r['trouble'].sum())} flagged at the end")
+    save("run_rotor_f4_pp_kx_32", uh0=pack(r0["uh"]), Bx0=r0["Bx"], By0=r0["By"], domain=dom_r, bc=1, M=10.0,
+         tend=0.01, Ck=1.0, uh=pack(r["uh"]), Bx=r["Bx"], By=r["By"], t=r["t"], count=r["count"],
+         umax=r["umax"], totaldiv=r["totaldiv"], pmin=r["pmin"], trouble=r["trouble"])
 
 
 if __name__ == "__main__":

@@ -104,9 +104,11 @@ def test_dt_and_diagnostics(pkg):
     assert abs(d["totaldiv"] - float(g["totaldiv"])) < 1e-12 * float(g["totaldiv"])
 
 
-RUNS = [("run_f4_16", 4, 0), ("run_f2_16", 2, 0), ("run_f1_16", 1, 0), ("run_f3_16", 3, 0),
-        ("run_f4_pp_16_outflow", 4, 1), ("run_f4_pp_32", 4, 1), ("run_f2_32", 2, 0),
-        ("run_rotor_f4_pp_32", 4, 1)]
+RUNS = [("run_f4_16", 4, 0, 1), ("run_f2_16", 2, 0, 1), ("run_f1_16", 1, 0, 1), ("run_f3_16", 3, 0, 1),
+        ("run_f4_pp_16_outflow", 4, 1, 1), ("run_f4_pp_32", 4, 1, 1), ("run_f2_32", 2, 0, 1),
+        ("run_rotor_f4_pp_32", 4, 1, 1),
+        # TVB gated by KXRCF_Detector (Ck = 1)
+        ("run_f2_kx_32", 2, 0, 2), ("run_rotor_f4_pp_kx_32", 4, 1, 2)]
 
 
 # The rotor carries B = (B0, 0) in most of the domain: B is parallel to the x
@@ -117,14 +119,26 @@ RUNS = [("run_f4_16", 4, 0), ("run_f2_16", 2, 0), ("run_f1_16", 1, 0), ("run_f3_
 # into ~1e-8 relative changes of the limiter's eigenvectors.  Its whole-run bar
 # is therefore 1e-9; on identical inputs every RK stage of the same run meets
 # 1e-12 (test_rotor_per_stage_identical_inputs).
-RUN_TOL = {"run_rotor_f4_pp_32": 1e-9}
+RUN_TOL = {"run_rotor_f4_pp_32": 1e-9, "run_rotor_f4_pp_kx_32": 1e-9}
 
 
-@pytest.mark.parametrize("name,ft,pp", RUNS)
-def test_rk3_fused(pkg, name, ft, pp):
+@pytest.mark.parametrize("name", ["kxrcf_16", "kxrcf_rotor_32"])
+def test_kxrcf(pkg, name):
+    """KXRCF_Detector flags, cell by cell, against the reference (P6 build)"""
+    g = golden(name)
+    s, uh = solver_from(pkg, g, Ck=float(g["Ck"]))
+    s.upload(uh, g["Bx"], g["By"])
+    n = s.KXRCF_Detector()
+    flags = s.debug_get(pkg.TROUBLE)
+    assert n == int(g["trouble"].sum())
+    assert np.array_equal(flags.astype(np.int32), g["trouble"])
+
+
+@pytest.mark.parametrize("name,ft,pp,ind", RUNS)
+def test_rk3_fused(pkg, name, ft, pp, ind):
     """mhd2d_step (fused kernels, device-side dt) against the reference run"""
     g = golden(name)
-    s, uh0 = solver_from(pkg, g, uh_key="uh0", ft=ft, pp=pp)
+    s, uh0 = solver_from(pkg, g, uh_key="uh0", ft=ft, pp=pp, indicator=ind, Ck=float(g.get("Ck", 1.0)))
     s.upload(uh0, g["Bx0"], g["By0"])
     t, n = s.RK3(float(g["tend"]))
     assert n == int(g["count"])
@@ -135,6 +149,8 @@ def test_rk3_fused(pkg, name, ft, pp):
     assert rel_l2(Bx, g["Bx"]) < tol and rel_l2(By, g["By"]) < tol
     d = s.diagnostics()
     assert d["totaldiv"] < 1e-12
+    if "trouble" in g:      # the last KXRCF call of the run flags the same cells
+        assert np.array_equal(s.debug_get(pkg.TROUBLE).astype(np.int32), g["trouble"])
 
 
 @pytest.mark.parametrize("name,ft,pp", [("run_f4_pp_32", 4, 1), ("run_f4_pp_16_outflow", 4, 1)])

@@ -94,16 +94,53 @@ def test_dt_and_diagnostics(oracle_mod):
         assert s[k] == float(g[k]), k
 
 
-@pytest.mark.parametrize("name,ft,pp", [("run_f4_16", 4, 0), ("run_f2_16", 2, 0), ("run_f1_16", 1, 0),
-                                        ("run_f3_16", 3, 0), ("run_f4_pp_16_outflow", 4, 1),
-                                        ("run_f4_pp_32", 4, 1), ("run_f2_32", 2, 0),
-                                        ("run_rotor_f4_pp_32", 4, 1)])
-def test_rk3_run(oracle_mod, name, ft, pp):
+@pytest.mark.parametrize("name", ["kxrcf_16", "kxrcf_rotor_32"])
+def test_kxrcf(oracle_mod, name):
+    O = oracle_mod
+    g = golden(name)
+    uh = unpack(g["uh"])
+    o = O.Oracle(uh.shape[0] - 2, uh.shape[1] - 2, domain=tuple(g["domain"]), Ck=float(g["Ck"]))
+    o.load(uh, g["Bx"], g["By"])
+    o.KXRCF_Detector()
+    assert 0 < int(g["trouble"].sum()) < g["trouble"].size
+    assert np.array_equal(o.trouble, g["trouble"])
+
+
+def test_kxrcf_closed_form(oracle_mod):
+    """Piecewise-constant states: a uniform flow has no inflow jump (no
+    flags); a density step advected in +x flags exactly the column of cells
+    whose left (inflow) face carries the jump, KXRCF_Detector.f90:62-68."""
+    O = oracle_mod
+    N = 8
+    o = O.Oracle(N, N, domain=(0.0, 1.0, 0.0, 1.0), Ck=1.0)
+    uh = np.zeros((N + 2, N + 2, 10, 8), order="F")
+    uh[:, :, 0, 0] = 1.0
+    uh[:, :, 0, 1] = 0.5          # rho u > 0: inflow through every left face
+    uh[:, :, 0, 4] = 2.0
+    Bx, By = np.zeros((N + 1, N, 3), order="F"), np.zeros((N, N + 1, 3), order="F")
+    o.load(uh, Bx, By)
+    o.KXRCF_Detector()
+    assert o.trouble.sum() == 0
+    uh[5:, :, 0, 0] = 2.0         # cells i >= 5 (1-based) are denser: jump on the left face of i = 5
+    uh[5:, :, 0, 1] = 1.0         # same velocity
+    o.load(uh, Bx, By)
+    o.KXRCF_Detector()
+    expect = np.zeros((N, N), dtype=np.int32)
+    expect[4, :] = 1
+    assert np.array_equal(o.trouble, expect)
+
+
+@pytest.mark.parametrize("name,ft,pp,ind", [("run_f4_16", 4, 0, 1), ("run_f2_16", 2, 0, 1), ("run_f1_16", 1, 0, 1),
+                                            ("run_f3_16", 3, 0, 1), ("run_f4_pp_16_outflow", 4, 1, 1),
+                                            ("run_f4_pp_32", 4, 1, 1), ("run_f2_32", 2, 0, 1),
+                                            ("run_rotor_f4_pp_32", 4, 1, 1),
+                                            ("run_f2_kx_32", 2, 0, 2), ("run_rotor_f4_pp_kx_32", 4, 1, 2)])
+def test_rk3_run(oracle_mod, name, ft, pp, ind):
     O = oracle_mod
     g = golden(name)
     uh0 = unpack(g["uh0"])
     o = O.Oracle(uh0.shape[0] - 2, uh0.shape[1] - 2, flux_type=ft, bc=int(g["bc"]),
-                 domain=tuple(g["domain"]), M=float(g["M"]), pp=pp)
+                 domain=tuple(g["domain"]), M=float(g["M"]), pp=pp, indicator=ind)
     o.load(uh0, g["Bx0"], g["By0"])
     o.rk3_prologue()
     while o.rk3_step(float(g["tend"])):
