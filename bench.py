@@ -164,8 +164,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--form", choices=["fp64", "mixed"], default="mixed",
-                    help="the form the headline value is measured on (the plain FP64 form always runs too)")
+    ap.add_argument("--form", choices=["fp64", "mixed", "mixed_nn"], default="mixed_nn",
+                    help="the form the headline value is measured on (the plain FP64 form always runs too); "
+                         "mixed_nn = mixed precision + the learned troubled-cell indicator")
     ap.add_argument("--all-cells", type=int, default=1,
                     help="also time the plain form with TVB on every cell (the reference as shipped)")
     ap.add_argument("--error", type=int, default=1, help="report mixed-vs-FP64 L2 errors (downloads both states)")
@@ -187,9 +188,17 @@ def main():
         counters_run(pkg, N, dom)
         return
 
+    nn = None
+    if args.form == "mixed_nn":
+        z = np.load(pkg.WEIGHTS_PATH, allow_pickle=False)
+        nn = (pkg.load_indicator_weights(), float(z["mlp_lo"]), float(z["mlp_hi"]))
+
     def run(prec, ind, steps, warmup, profile_steps):
+        band = (nn[1], nn[2]) if nn else (0.2, 0.8)
         s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=ind,
-                       precision=prec, Ck=1.0)
+                       precision=prec, Ck=1.0, mlp_band=band)
+        if nn:
+            s.mlp_set_weights(nn[0])
         s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
         s.rk3_prologue()
         for _ in range(warmup):
@@ -203,6 +212,7 @@ def main():
         state = s.download() if args.error else None
         diag = s.diagnostics()
         tnow = s.get_time()[0]
+        istats = s.indicator_stats() if (nn and ind == pkg.IND_MLP) else None
         prof = None
         if profile_steps:
             s.profile(True)
@@ -212,14 +222,16 @@ def main():
             s.profile(False)
         s.close()
         return dict(secs=secs, state=state, diag=diag, t=tnow, prof=prof, ms=1e3 * secs / steps,
-                    value=DOF_PER_CELL * N * N * steps / secs)
+                    value=DOF_PER_CELL * N * N * steps / secs, istats=istats)
 
     # the plain method of the workload: FP64, TVB gated by KXRCF (Ck = 1)
     plan = [("fp64", pkg.FP64, pkg.IND_KXRCF)]
     if args.all_cells:
         plan.append(("fp64_all_cells", pkg.FP64, pkg.IND_ALL))
-    if args.form == "mixed":
+    if args.form in ("mixed", "mixed_nn"):
         plan.append(("mixed", pkg.MIXED, pkg.IND_KXRCF))
+    if args.form == "mixed_nn":
+        plan.append(("mixed_nn", pkg.MIXED, pkg.IND_MLP))
     results = {}
     for name, prec, ind in plan:
         r = run(prec, ind, args.steps, args.warmup, args.profile_steps)
@@ -237,7 +249,7 @@ def main():
     dom_k = max(prof, key=lambda k: prof[k][0])
     ms_tot, launches = prof[dom_k]
     avg_ms = ms_tot / max(launches, 1)
-    bytes_launch = alg_bytes(args.form)[dom_k] * N * N
+    bytes_launch = alg_bytes("fp64" if args.form == "fp64" else "mixed")[dom_k] * N * N
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     step_gbs = STEP_ALG_BYTES_PER_CELL * N * N / (ms_per_step * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline()
@@ -253,18 +265,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "f32+f64",
+        "dtype": "f64" if args.form == "fp64" else "f32+f64" + ("+bf16" if args.form == "mixed_nn" else ""),
         "data": "synthetic",
         "config": {"workload": f"perturbed Orszag-Tang {N}x{N}, P2 DG (48 DOF/cell), HLLD, TVB M=50 gated by "
                                f"KXRCF (Ck=1), positivity, SSP-RK3, CFL 0.2", "N": N, "form": args.form,
-                   "indicator": "KXRCF", "totaldiv": d["totaldiv"], "pmin": d["pmin"],
+                   "indicator": "learned MLP, KXRCF in the band" if args.form == "mixed_nn" else "KXRCF",
+                   "totaldiv": d["totaldiv"], "pmin": d["pmin"],
                    "fp64_value": results["fp64"]["value"], "fp64_ms_per_step": results["fp64"]["ms"],
                    "speedup_vs_fp64": value / results["fp64"]["value"],
                    "rel_l2_vs_fp64": errors,
                    "fp64_all_cells_value": results.get("fp64_all_cells", {}).get("value"),
-                   "fp64_all_cells_ms_per_step": results.get("fp64_all_cells", {}).get("ms")},
+                   "fp64_all_cells_ms_per_step": results.get("fp64_all_cells", {}).get("ms"),
+                   "mixed_kxrcf_value": results.get("mixed", {}).get("value"),
+                   "indicator_stats": head.get("istats"),
+                   "mlp_band": [nn[1], nn[2]] if nn else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_k, args.form, N),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_k, "fp64" if args.form == "fp64" else "mixed", N),
                      "kernel": dom_k, "alg_bytes_per_launch": bytes_launch,
                      "kernel_avg_ms": avg_ms, "step_achieved": step_gbs,
                      "step_frac": step_gbs / HBM_PEAK_GBS},

@@ -34,7 +34,15 @@ EXPORTS = (
     "mhd2d_div_free_balsara", "mhd2d_calculate_dt", "mhd2d_diagnostics", "mhd2d_rk3_prologue",
     "mhd2d_step", "mhd2d_get_time", "mhd2d_set_time", "mhd2d_debug_get",
     "mhd2d_profile_enable", "mhd2d_profile_get",
+    "mhd2d_mlp_set_weights", "mhd2d_mlp_probability", "mhd2d_indicator_stats",
+    "mhd2d_trainer_create", "mhd2d_trainer_destroy", "mhd2d_trainer_last_error", "mhd2d_trainer_collect",
+    "mhd2d_trainer_size", "mhd2d_trainer_clear", "mhd2d_trainer_get_data", "mhd2d_trainer_set_data",
+    "mhd2d_trainer_grad", "mhd2d_trainer_fit",
 )
+MLP_NPARAM = 8129        # W1[60][64], b1[64], W2[64][64], b2[64], w3[64], b3
+MLP_NIN = 60
+WEIGHTS_PATH = os.path.join(HERE, "indicator_weights.npz")
+STATS_KEYS = ("cells", "label", "kxrcf", "decision", "band", "missed", "over", "over_changed")
 KERNEL_CLASSES = ("xface", "yface", "vertex", "cell", "ct", "limit", "setbc", "dt", "kxrcf", "diag")
 
 
@@ -66,6 +74,7 @@ def load_library(path: str = LIB_PATH):
         raise MHD2DError(f"{path} is not built (run __graft_entry__.build() or make -C {HERE})")
     L = C.CDLL(path)
     P, dp, i32, i64p = C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int64)
+    fp = C.POINTER(C.c_float)
     sig = {
         "mhd2d_create": ([C.POINTER(Config), C.POINTER(C.c_void_p)], C.c_int),
         "mhd2d_destroy": ([P], C.c_int),
@@ -91,6 +100,19 @@ def load_library(path: str = LIB_PATH):
         "mhd2d_debug_get": ([P, i32, dp], C.c_int),
         "mhd2d_profile_enable": ([P, i32], C.c_int),
         "mhd2d_profile_get": ([P, dp, i64p, i32], C.c_int),
+        "mhd2d_mlp_set_weights": ([P, fp, i32], C.c_int),
+        "mhd2d_mlp_probability": ([P, fp], C.c_int),
+        "mhd2d_indicator_stats": ([P, i64p], C.c_int),
+        "mhd2d_trainer_create": ([C.c_int64, i32, C.POINTER(C.c_void_p)], C.c_int),
+        "mhd2d_trainer_destroy": ([P], C.c_int),
+        "mhd2d_trainer_last_error": ([P], C.c_char_p),
+        "mhd2d_trainer_collect": ([P, P, C.c_double, C.c_uint64, i64p, i64p], C.c_int),
+        "mhd2d_trainer_size": ([P, i64p, i64p], C.c_int),
+        "mhd2d_trainer_clear": ([P], C.c_int),
+        "mhd2d_trainer_get_data": ([P, C.c_int64, C.c_int64, fp, fp, i64p], C.c_int),
+        "mhd2d_trainer_set_data": ([P, fp, fp, C.c_int64], C.c_int),
+        "mhd2d_trainer_grad": ([P, fp, C.c_int64, i32, C.c_float, fp, fp], C.c_int),
+        "mhd2d_trainer_fit": ([P, fp, i32, i32, C.c_float, C.c_float, C.c_uint64, fp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -102,6 +124,23 @@ def load_library(path: str = LIB_PATH):
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _i64p(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def load_indicator_weights(path=WEIGHTS_PATH):
+    """the trained indicator parameters shipped with the package (FP32)"""
+    with np.load(path, allow_pickle=False) as z:
+        p = np.ascontiguousarray(z["params"], dtype=np.float32)
+    if p.size != MLP_NPARAM:
+        raise MHD2DError(f"{path}: {p.size} parameters, expected {MLP_NPARAM}")
+    return p
 
 
 class Solver:
@@ -236,6 +275,21 @@ class Solver:
                   "profile_get")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
 
+    # --- learned indicator ----------------------------------------------
+    def mlp_set_weights(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._chk(self.L.mhd2d_mlp_set_weights(self.h, _fp(p), int(p.size)), "mlp_set_weights")
+
+    def mlp_probability(self):
+        p = np.zeros((self.Nx, self.Ny), dtype=np.float32, order="F")
+        self._chk(self.L.mhd2d_mlp_probability(self.h, _fp(p)), "mlp_probability")
+        return p
+
+    def indicator_stats(self):
+        out = np.zeros(8, dtype=np.int64)
+        self._chk(self.L.mhd2d_indicator_stats(self.h, _i64p(out)), "indicator_stats")
+        return dict(zip(STATS_KEYS, (int(v) for v in out)))
+
     def RK3(self, tend, max_steps=None):
         """RK3.f90:5-174 without frame output: prologue, then steps while t < tend."""
         self.rk3_prologue()
@@ -245,3 +299,72 @@ class Solver:
             t = self.step(tend, sync=True)
             n += 1
         return t, n
+
+
+class Trainer:
+    """Device dataset + FP32 training of the indicator MLP (mhd2d_trainer_*)."""
+
+    def __init__(self, capacity, max_batch=4096):
+        self.L = load_library()
+        h = C.c_void_p()
+        rc = self.L.mhd2d_trainer_create(int(capacity), int(max_batch), C.byref(h))
+        if rc != 0:
+            raise MHD2DError(f"mhd2d_trainer_create failed with status {rc}")
+        self.h = h
+        self.max_batch = int(max_batch)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mhd2d_trainer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.mhd2d_trainer_last_error(self.h).decode(errors="replace")
+            raise MHD2DError(f"{what}: status {rc}: {msg}")
+
+    def collect(self, solver, neg_keep, seed):
+        n, npos = C.c_int64(), C.c_int64()
+        self._chk(self.L.mhd2d_trainer_collect(self.h, solver.h, float(neg_keep), int(seed), C.byref(n),
+                                               C.byref(npos)), "collect")
+        return n.value, npos.value
+
+    def size(self):
+        n, npos = C.c_int64(), C.c_int64()
+        self._chk(self.L.mhd2d_trainer_size(self.h, C.byref(n), C.byref(npos)), "size")
+        return n.value, npos.value
+
+    def clear(self):
+        self._chk(self.L.mhd2d_trainer_clear(self.h), "clear")
+
+    def get_data(self, offset=0, count=None):
+        n, _ = self.size()
+        count = n - offset if count is None else count
+        X = np.zeros((count, MLP_NIN), dtype=np.float32)
+        Y = np.zeros(count, dtype=np.float32)
+        idx = np.zeros(count, dtype=np.int64)
+        self._chk(self.L.mhd2d_trainer_get_data(self.h, offset, count, _fp(X), _fp(Y), _i64p(idx)), "get_data")
+        return X, Y, idx
+
+    def set_data(self, X, Y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        self._chk(self.L.mhd2d_trainer_set_data(self.h, _fp(X), _fp(Y), int(Y.size)), "set_data")
+
+    def grad(self, params, offset, count, pos_weight=1.0):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._chk(self.L.mhd2d_trainer_grad(self.h, _fp(p), int(offset), int(count), float(pos_weight), _fp(g),
+                                            _fp(loss)), "grad")
+        return float(loss[0]), g
+
+    def fit(self, params, steps, batch, lr, pos_weight=1.0, seed=0):
+        p = np.array(params, dtype=np.float32, copy=True)
+        losses = np.zeros(steps, dtype=np.float32)
+        self._chk(self.L.mhd2d_trainer_fit(self.h, _fp(p), int(steps), int(batch), float(lr), float(pos_weight),
+                                           int(seed), _fp(losses)), "fit")
+        return p, losses

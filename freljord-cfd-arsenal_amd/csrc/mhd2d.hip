@@ -362,6 +362,7 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
   for (int b : bcs)
     if (b != 1 && b != 2) return -1;
   if (cfg->precision != MHD2D_FP64 && cfg->precision != MHD2D_MIXED) return -1;
+  if (cfg->indicator < MHD2D_IND_NONE || cfg->indicator > MHD2D_IND_MLP) return -1;
   mhd2d_ctx* ctx = (mhd2d_ctx*)calloc(1, sizeof(mhd2d_ctx));
   if (!ctx) return -1;
   new (&ctx->prof_ev) decltype(ctx->prof_ev)();
@@ -395,7 +396,9 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
       {(void**)&ctx->trouble, (size_t)g.Nx * g.Ny * sizeof(int)},
       {(void**)&ctx->tmask, (size_t)g.Nx * g.Ny * sizeof(int)},
       {(void**)&ctx->tlist, (size_t)g.Nx * g.Ny * sizeof(int)},
+      {(void**)&ctx->blist, (size_t)g.Nx * g.Ny * sizeof(int)},
       {(void**)&ctx->trec, (size_t)32 * g.Nx * g.Ny * sizeof(double)},
+      {(void**)&ctx->prob, (size_t)g.Nx * g.Ny * sizeof(float)},
       {(void**)&ctx->sc, sizeof(DevScal)},
       {(void**)&ctx->partial, (size_t)ctx->nblk_cells * sizeof(double)},
       {(void**)&ctx->icp, sizeof(ICPar)},
@@ -433,7 +436,7 @@ int mhd2d_destroy(mhd2d_ctx* ctx) {
   ctx->ev_pool.~vector();
   void* ps[] = {ctx->S, ctx->U0, ctx->UI, ctx->UII, ctx->T, ctx->du, ctx->F, ctx->F0, ctx->FI, ctx->FII,
                 ctx->Ft, ctx->dEz, ctx->FX, ctx->FY, ctx->EZX, ctx->EZY, ctx->EZV, ctx->trouble, ctx->tmask,
-                ctx->tlist, ctx->trec, ctx->sc, ctx->partial, ctx->icp};
+                ctx->tlist, ctx->blist, ctx->trec, ctx->prob, ctx->mlpw, ctx->sc, ctx->partial, ctx->icp};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

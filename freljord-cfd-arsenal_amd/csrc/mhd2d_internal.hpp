@@ -33,7 +33,7 @@ struct DevScal {
   long long count;
   long long nflag;
   unsigned int nlist;   // length of the troubled-cell list of the current limiter pass
-  unsigned int pad1;
+  unsigned int nband;   // length of the indicator's confidence-band list
   int active;
   int pad;
 };
@@ -94,6 +94,16 @@ struct ICPar {
   double kx[16], ky[16], phu[16], phv[16];
 };
 
+// the indicator MLP 60-64-64-1: parameter vector layout (FP32)
+constexpr int MLP_NIN = 60;
+constexpr int MLP_OFF_W1 = 0;                        // [60][64], row = input
+constexpr int MLP_OFF_B1 = MLP_OFF_W1 + 60 * 64;     // [64]
+constexpr int MLP_OFF_W2 = MLP_OFF_B1 + 64;          // [64][64]
+constexpr int MLP_OFF_B2 = MLP_OFF_W2 + 64 * 64;     // [64]
+constexpr int MLP_OFF_W3 = MLP_OFF_B2 + 64;          // [64]
+constexpr int MLP_OFF_B3 = MLP_OFF_W3 + 64;          // [1]
+constexpr int MLP_NPARAM = MLP_OFF_B3 + 1;           // 8129
+
 // kernel classes for mhd2d_profile_get
 enum { KC_XFACE = 0, KC_YFACE, KC_VERTEX, KC_CELL, KC_CT, KC_LIMIT, KC_SETBC, KC_DT, KC_KXRCF, KC_DIAG, KC_N };
 
@@ -117,7 +127,10 @@ struct mhd2d_ctx {
   int* trouble;        // detector decision per interior cell (1 = TVB applied)
   int* tmask;          // TVB change bits per interior cell (0 = unchanged)
   int* tlist;          // interior-cell indices the detector flagged (length sc->nlist)
+  int* blist;          // interior-cell indices inside the indicator's confidence band (sc->nband)
   double* trec;        // modified edge values of changed cells: 32 planes over the interior cells
+  void* mlpw;          // indicator MLP weights in MFMA operand order (mhd2d_mlp.hip)
+  float* prob;         // indicator MLP probability per interior cell
   mhd2d_impl::DevScal* sc;
   double* partial;
   mhd2d_impl::ICPar* icp;
@@ -198,6 +211,10 @@ int launch_ct(mhd2d_ctx* ctx, int mode, const double* F, const double* A, const 
 int launch_axpy(mhd2d_ctx* ctx, double* out, const double* X, double a, const double* Y, double b,
                 const double* Z, double dt, long long n);                                     // mhd2d_cell.hip
 int launch_kxrcf(mhd2d_ctx* ctx, const double* U, DevScal* sc);                               // mhd2d_limit.hip
+int launch_tvb_list(mhd2d_ctx* ctx, const double* in);                                       // mhd2d_limit.hip
+int launch_detect_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc = nullptr);              // mhd2d_limit.hip
+int launch_tvb_all(mhd2d_ctx* ctx, const double* in, int* kx);                               // mhd2d_limit.hip
+int launch_mlp_detect(mhd2d_ctx* ctx, const double* in, DevScal* sc, bool list, float* prob);  // mhd2d_mlp.hip
 int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool tvb, bool pp, bool div,
                  DevScal* sc);                                                                // mhd2d_limit.hip
 

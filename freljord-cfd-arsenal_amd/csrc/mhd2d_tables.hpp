@@ -91,6 +91,38 @@ __host__ __device__ constexpr double phiTr(int side, int b, double s, double p2,
                    : (b == 0 ? 1.0 : b == 1 ? s : b == 2 ? -1.0 : b == 3 ? p2 : b == 4 ? -s : b == 5 ? 2.0 / 3.0
                       : b == 6 ? p3 : b == 7 ? -p2 : b == 8 ? t23 : -2.0 / 5.0);
 }
+// phiTr(side, b, s) written in the basis {1, s, P2(s), P3(s)}: coefficient k
+__host__ __device__ constexpr double trk(int side, int b, int k) {
+  const double t23 = 2.0 / 3.0, t25 = 2.0 / 5.0;
+  const double sg = (side == 0 || side == 2) ? 1.0 : -1.0;   // the face's normal coordinate, +-1
+  if (side < 2) {   // x faces: normal xi = sg, tangential s = eta
+    switch (b) {
+      case 0: return k == 0 ? 1.0 : 0.0;
+      case 1: return k == 0 ? sg : 0.0;
+      case 2: return k == 1 ? 1.0 : 0.0;
+      case 3: return k == 0 ? t23 : 0.0;
+      case 4: return k == 1 ? sg : 0.0;
+      case 5: return k == 2 ? 1.0 : 0.0;
+      case 6: return k == 0 ? sg * t25 : 0.0;
+      case 7: return k == 1 ? t23 : 0.0;
+      case 8: return k == 2 ? sg : 0.0;
+      default: return k == 3 ? 1.0 : 0.0;
+    }
+  }
+  switch (b) {      // y faces: normal eta = sg, tangential s = xi
+    case 0: return k == 0 ? 1.0 : 0.0;
+    case 1: return k == 1 ? 1.0 : 0.0;
+    case 2: return k == 0 ? sg : 0.0;
+    case 3: return k == 2 ? 1.0 : 0.0;
+    case 4: return k == 1 ? sg : 0.0;
+    case 5: return k == 0 ? t23 : 0.0;
+    case 6: return k == 3 ? 1.0 : 0.0;
+    case 7: return k == 2 ? sg : 0.0;
+    case 8: return k == 1 ? t23 : 0.0;
+    default: return k == 0 ? sg * t25 : 0.0;
+  }
+}
+
 // vertex values phiRU/LU/RD/LD (get_basis.f90:5-47); v: 0 RU, 1 LU, 2 RD, 3 LD
 __host__ __device__ constexpr double phiV(int v, int b) {
   const double t23 = 2.0 / 3.0, t25 = 2.0 / 5.0;

@@ -108,6 +108,51 @@ int mhd2d_debug_get(mhd2d_ctx *ctx, int32_t which, double *host);
 int mhd2d_profile_enable(mhd2d_ctx *ctx, int32_t on);
 int mhd2d_profile_get(mhd2d_ctx *ctx, double *ms, int64_t *launches, int32_t nclass);
 
+/* ---- learned troubled-cell indicator (no counterpart in the reference) ----
+   A 60-64-64-1 MLP (GELU hidden layers, sigmoid output) over a cell's and its
+   four neighbours' normalised modal coefficients (density and total energy,
+   6 modes each).  With indicator = MHD2D_IND_MLP the limiter pass applies TVB
+   where p >= mlp_hi, skips cells with p < mlp_lo and asks KXRCF_Detector in
+   between.  Parameters are one FP32 vector of MHD2D_MLP_NPARAM values:
+   W1[60][64], b1[64], W2[64][64], b2[64], w3[64], b3 (W rows = inputs).
+   Inference runs in BF16 on the matrix cores with FP32 accumulation. */
+#define MHD2D_MLP_NPARAM 8129
+#define MHD2D_MLP_NIN 60
+int mhd2d_mlp_set_weights(mhd2d_ctx *ctx, const float *params, int32_t n);
+/* indicator probability of every interior cell of the current state;
+   host_p is (Nx, Ny) column-major (may be NULL) */
+int mhd2d_mlp_probability(mhd2d_ctx *ctx, float *host_p);
+/* indicator statistics on the current state, out[8] = { cells, label
+   (KXRCF flags AND TVB changes), KXRCF flags, MLP decisions (TVB applied),
+   cells in the confidence band, missed (label and not applied), over-flagged
+   (applied and not label), over-flagged that TVB changes (applied, not
+   KXRCF, TVB changes) } */
+int mhd2d_indicator_stats(mhd2d_ctx *ctx, int64_t *out);
+
+/* FP32 training of the indicator on the GPU.  A trainer owns a device
+   dataset of (60 features, label) rows, collected from solver states with
+   the plain method's own decision as the label, and the training buffers. */
+typedef struct mhd2d_trainer mhd2d_trainer;
+int mhd2d_trainer_create(int64_t capacity, int32_t max_batch, mhd2d_trainer **out);
+int mhd2d_trainer_destroy(mhd2d_trainer *tr);
+const char *mhd2d_trainer_last_error(const mhd2d_trainer *tr);
+/* append the labelled cells of ctx's current state (every positive, each
+   negative with probability neg_keep) */
+int mhd2d_trainer_collect(mhd2d_trainer *tr, mhd2d_ctx *ctx, double neg_keep, uint64_t seed, int64_t *n_total,
+                          int64_t *n_pos);
+int mhd2d_trainer_size(mhd2d_trainer *tr, int64_t *n, int64_t *npos);
+int mhd2d_trainer_clear(mhd2d_trainer *tr);
+int mhd2d_trainer_get_data(mhd2d_trainer *tr, int64_t offset, int64_t count, float *X, float *Y, int64_t *idx);
+int mhd2d_trainer_set_data(mhd2d_trainer *tr, const float *X, const float *Y, int64_t n);
+/* gradient and loss of params over rows [offset, offset + count) (loss =
+   mean over rows of w * binary cross-entropy, w = pos_weight for label 1) */
+int mhd2d_trainer_grad(mhd2d_trainer *tr, const float *params, int64_t offset, int32_t count, float pos_weight,
+                       float *grad, float *loss);
+/* Adam (beta 0.9, 0.999, eps 1e-8) for `steps` batches of `batch` random
+   rows; params in/out; losses[steps] (may be NULL) */
+int mhd2d_trainer_fit(mhd2d_trainer *tr, float *params, int32_t steps, int32_t batch, float lr, float pos_weight,
+                      uint64_t seed, float *losses);
+
 #ifdef __cplusplus
 }
 #endif
