@@ -57,7 +57,8 @@ def alg_bytes(form):
 
 
 STEP_ALG_BYTES_PER_CELL = 2688     # 42 independent FP64 values x 8 state transfers per SSP-RK3 step
-# kernel-name prefix of each class in the rocprofv3 traces
+# kernel of each class in the traffic summary (the limiter class also runs
+# the detector and the sparse TVB pass, whose traffic is listed on its own)
 KERNEL_SYMBOL = {"xface": "k_xface", "yface": "k_yface", "vertex": "k_vertex", "cell": "k_cell", "ct": "k_ct",
                  "limit": "k_limit", "dt": "k_dt", "kxrcf": "k_kxrcf", "diag": "k_diag", "setbc": "k_setbc"}
 
@@ -115,7 +116,8 @@ def cpu_baseline(N=256, steps=2):
 
 
 def counters_run(pkg, N, dom):
-    """Calibration dispatch plus two steps of each form, for rocprofv3 --pmc.
+    """Calibration dispatch plus two steps of each benchmarked form (plain
+    FP64, mixed, mixed + learned indicator), for rocprofv3 --pmc.
 
     The calibration kernel is k_axpy of the per-subroutine stage update
     (mhd2d_rk_stage 1 with dt = 0): over the 52 state planes it reads two
@@ -130,9 +132,13 @@ def counters_run(pkg, N, dom):
     n = 52 * (N + 2) * P
     print(f"[counters] k_axpy over {n} doubles: reads {16 * n} B, writes {8 * n} B", file=sys.stderr, flush=True)
     s.close()
-    for name, prec in (("fp64", pkg.FP64), ("mixed", pkg.MIXED)):
-        s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=pkg.IND_ALL,
-                       precision=prec)
+    z = np.load(pkg.WEIGHTS_PATH, allow_pickle=False)
+    params, band = pkg.load_indicator_weights(), (float(z["mlp_lo"]), float(z["mlp_hi"]))
+    for name, prec, ind in (("fp64", pkg.FP64, pkg.IND_KXRCF), ("mixed", pkg.MIXED, pkg.IND_KXRCF),
+                            ("mixed_nn", pkg.MIXED, pkg.IND_MLP)):
+        s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=ind,
+                       precision=prec, Ck=1.0, mlp_band=band)
+        s.mlp_set_weights(params)
         s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
         s.rk3_prologue()
         s.step()
@@ -280,7 +286,7 @@ def main():
                    "indicator_stats": head.get("istats"),
                    "mlp_band": [nn[1], nn[2]] if nn else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_k, "fp64" if args.form == "fp64" else "mixed", N),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_k, args.form, N),
                      "kernel": dom_k, "alg_bytes_per_launch": bytes_launch,
                      "kernel_avg_ms": avg_ms, "step_achieved": step_gbs,
                      "step_frac": step_gbs / HBM_PEAK_GBS},

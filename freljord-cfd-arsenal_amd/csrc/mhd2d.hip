@@ -397,6 +397,7 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
       {(void**)&ctx->tmask, (size_t)g.Nx * g.Ny * sizeof(int)},
       {(void**)&ctx->tlist, (size_t)g.Nx * g.Ny * sizeof(int)},
       {(void**)&ctx->blist, (size_t)g.Nx * g.Ny * sizeof(int)},
+      {(void**)&ctx->screen, (size_t)g.Nx * g.Ny * sizeof(int)},
       {(void**)&ctx->trec, (size_t)32 * g.Nx * g.Ny * sizeof(double)},
       {(void**)&ctx->prob, (size_t)g.Nx * g.Ny * sizeof(float)},
       {(void**)&ctx->sc, sizeof(DevScal)},
@@ -436,7 +437,7 @@ int mhd2d_destroy(mhd2d_ctx* ctx) {
   ctx->ev_pool.~vector();
   void* ps[] = {ctx->S, ctx->U0, ctx->UI, ctx->UII, ctx->T, ctx->du, ctx->F, ctx->F0, ctx->FI, ctx->FII,
                 ctx->Ft, ctx->dEz, ctx->FX, ctx->FY, ctx->EZX, ctx->EZY, ctx->EZV, ctx->trouble, ctx->tmask,
-                ctx->tlist, ctx->blist, ctx->trec, ctx->prob, ctx->mlpw, ctx->sc, ctx->partial, ctx->icp};
+                ctx->tlist, ctx->blist, ctx->screen, ctx->trec, ctx->prob, ctx->mlpw, ctx->sc, ctx->partial, ctx->icp};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -784,7 +785,7 @@ int mhd2d_step(mhd2d_ctx* ctx, double tend, double* t_out) {
   if ((rc = launch_fluxes(ctx, ctx->S, true, sc))) return rc;
   if ((rc = launch_cell(ctx, 1, ctx->S, nullptr, nullptr, ctx->UI, sc, 0.0))) return rc;
   if ((rc = launch_ct(ctx, 1, ctx->F, nullptr, nullptr, ctx->FI, sc, 0.0))) return rc;
-  if ((rc = launch_limit(ctx, ctx->UI, ctx->T, ctx->FI, true, pp, true, sc))) return rc;
+  if ((rc = launch_limit(ctx, ctx->UI, ctx->T, ctx->FI, true, pp, true, sc, 1))) return rc;
   if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;          // RK3.f90:174
   std::swap(ctx->U0, ctx->S);   // U0 <- start-of-step state
   std::swap(ctx->S, ctx->T);    // S  <- limited stage-1 state
@@ -793,14 +794,14 @@ int mhd2d_step(mhd2d_ctx* ctx, double tend, double* t_out) {
   if ((rc = launch_fluxes(ctx, ctx->S, true, sc))) return rc;
   if ((rc = launch_cell(ctx, 2, ctx->S, ctx->UI, ctx->U0, ctx->UII, sc, 0.0))) return rc;
   if ((rc = launch_ct(ctx, 2, nullptr, ctx->FI, ctx->F0, ctx->FII, sc, 0.0))) return rc;
-  if ((rc = launch_limit(ctx, ctx->UII, ctx->T, ctx->FII, true, pp, true, sc))) return rc;
+  if ((rc = launch_limit(ctx, ctx->UII, ctx->T, ctx->FII, true, pp, true, sc, 2))) return rc;
   if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;          // RK3.f90:200 (stage 3 set_bc)
   std::swap(ctx->S, ctx->T);
   // stage 3: T = 1/3 U0 + 2/3 (UII + dt L(S)); S_new = limit(T)
   if ((rc = launch_fluxes(ctx, ctx->S, true, sc))) return rc;
   if ((rc = launch_cell(ctx, 3, ctx->S, ctx->UII, ctx->U0, ctx->T, sc, 0.0))) return rc;
   if ((rc = launch_ct(ctx, 3, nullptr, ctx->FII, ctx->F0, ctx->F, sc, 0.0))) return rc;
-  if ((rc = launch_limit(ctx, ctx->T, ctx->UI, ctx->F, true, pp, true, sc))) return rc;
+  if ((rc = launch_limit(ctx, ctx->T, ctx->UI, ctx->F, true, pp, true, sc, 3))) return rc;
   std::swap(ctx->S, ctx->UI);   // S <- new state (halos refreshed by the next step's set_bc); UI is free
   ctx->lh_done = false;
   if (t_out) {

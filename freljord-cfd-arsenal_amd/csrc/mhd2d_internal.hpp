@@ -33,7 +33,7 @@ struct DevScal {
   long long count;
   long long nflag;
   unsigned int nlist;   // length of the troubled-cell list of the current limiter pass
-  unsigned int nband;   // length of the indicator's confidence-band list
+  unsigned int nband;   // length of the indicator's candidate list (cells handed to KXRCF)
   int active;
   int pad;
 };
@@ -127,7 +127,8 @@ struct mhd2d_ctx {
   int* trouble;        // detector decision per interior cell (1 = TVB applied)
   int* tmask;          // TVB change bits per interior cell (0 = unchanged)
   int* tlist;          // interior-cell indices the detector flagged (length sc->nlist)
-  int* blist;          // interior-cell indices inside the indicator's confidence band (sc->nband)
+  int* blist;          // interior-cell indices the indicator hands to KXRCF (sc->nband)
+  int* screen;         // indicator: 1 where p >= mlp_lo (before the one-cell dilation)
   double* trec;        // modified edge values of changed cells: 32 planes over the interior cells
   void* mlpw;          // indicator MLP weights in MFMA operand order (mhd2d_mlp.hip)
   float* prob;         // indicator MLP probability per interior cell
@@ -214,8 +215,11 @@ int launch_kxrcf(mhd2d_ctx* ctx, const double* U, DevScal* sc);                 
 int launch_tvb_list(mhd2d_ctx* ctx, const double* in);                                       // mhd2d_limit.hip
 int launch_detect_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc = nullptr);              // mhd2d_limit.hip
 int launch_tvb_all(mhd2d_ctx* ctx, const double* in, int* kx);                               // mhd2d_limit.hip
-int launch_mlp_detect(mhd2d_ctx* ctx, const double* in, DevScal* sc, bool list, float* prob);  // mhd2d_mlp.hip
+int launch_mlp_screen(mhd2d_ctx* ctx, const double* in);   // indicator: prob[], screen, candidate list   // mhd2d_mlp.hip
+int launch_candidates_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc);  // KXRCF + TVB on the candidates  // mhd2d_mlp.hip
+// stage: 1..3 inside mhd2d_step (the learned indicator screens once, at
+// stage 1, and reuses the screen for stages 2 and 3); 0 for a standalone call
 int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool tvb, bool pp, bool div,
-                 DevScal* sc);                                                                // mhd2d_limit.hip
+                 DevScal* sc, int stage = 0);                                                                // mhd2d_limit.hip
 
 }  // namespace mhd2d_impl

@@ -389,7 +389,7 @@ int launch_tvb_all(mhd2d_ctx* ctx, const double* in, int* kx) {
 // over every cell that applies them, the positivity limiter and the
 // divergence-free reconstruction (k_limit)
 int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool tvb, bool pp, bool div,
-                 DevScal* sc) {
+                 DevScal* sc, int stage) {
   const Geo& g = ctx->g;
   const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
   const int ind = ctx->cfg.indicator;
@@ -402,9 +402,9 @@ int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F,
                          ctx->trouble, ctx->tmask, ctx->trec, g, ctx->cfg.M, ctx->cfg.Ck, kx_hk(g), sc);
       LAUNCHCHK(ctx);
     } else if (ind == MHD2D_IND_MLP) {
-      // learned indicator (KXRCF inside its confidence band), then TVB over the list
-      if ((rc = launch_mlp_detect(ctx, in, sc, true, ctx->prob))) return rc;
-      if ((rc = launch_tvb_list(ctx, in))) return rc;
+      // learned screen (once per step), then KXRCF + TVB on its candidates
+      if (stage <= 1 && (rc = launch_mlp_screen(ctx, in))) return rc;
+      if ((rc = launch_candidates_tvb(ctx, in, sc))) return rc;
     } else {
       if ((rc = launch_detect_tvb(ctx, in, sc))) return rc;
     }

@@ -2,11 +2,17 @@
 // (GELU, sigmoid output) over a cell's and its four neighbours' normalised
 // modal coefficients.
 //
-//   inference  k_mlp_detect: BF16 MFMA (v_mfma_f32_16x16x16_bf16) per wave of
-//              64 cells, FP32 accumulation; p >= hi marks the cell for TVB,
-//              p < lo clears it, cells in [lo, hi) fall back to KXRCF.  Writes
-//              the same flags and compacted list as the KXRCF detector, so the
-//              sparse TVB and apply passes are shared.
+//   screen     k_mlp_screen: BF16 MFMA (v_mfma_f32_16x16x16_bf16) per wave of
+//              64 cells, FP32 accumulation, p = P(the plain method limits the
+//              cell).  Cells with p >= mlp_lo, dilated by one cell, are the
+//              candidates; the rest are confidently smooth and skip the
+//              classical detector.  Inside mhd2d_step the screen runs once per
+//              step (on the first stage's limiter input) and serves all three
+//              stages: a feature moves by at most a fifth of a cell per step at
+//              the CFL numbers used, and the dilation covers it.
+//   detector   KXRCF_Detector on the candidates only, at every stage; its
+//              flags feed the same sparse TVB and apply passes as the plain
+//              method, so on candidate cells the decision is the plain one.
 //   training   FP32 kernels: dense forward, output + weighted cross-entropy,
 //              backward through GELU, weight gradients, Adam.
 //   labels     the plain method's own decision on the same state: KXRCF
@@ -151,12 +157,9 @@ struct MlpDev {
   float b1[64], b2[64], w3[64], b3;
 };
 
-__global__ __launch_bounds__(256) void k_mlp_detect(const double* __restrict__ U, int* __restrict__ trouble,
-                                                    int* __restrict__ tmask, int* __restrict__ list,
-                                                    int* __restrict__ blist,
+__global__ __launch_bounds__(256) void k_mlp_screen(const double* __restrict__ U, int* __restrict__ screen,
                                                     float* __restrict__ prob, const MlpDev* __restrict__ W,
-                                                    Geo g, float l10Mh2, float l10h, float lo, float hi,
-                                                    DevScal* lst) {
+                                                    Geo g, float l10Mh2, float l10h, float lo) {
   __shared__ __attribute__((aligned(16))) unsigned short sX[4][64 * XP];
   __shared__ float sZ[4][64];
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, grp = l >> 4, col = l & 15;
@@ -237,23 +240,47 @@ __global__ __launch_bounds__(256) void k_mlp_detect(const double* __restrict__ U
   __syncthreads();
   if (!valid) return;
   const float p = __builtin_amdgcn_rcpf(1.0f + __expf(-sZ[wv][l]));
-  if (prob) prob[t] = p;
-  // p >= hi: troubled; p < lo: not; in between: the classical detector
-  // decides (k_kxrcf_band, over the band list)
-  const bool flag = p >= hi, band = !flag && p >= lo;
-  trouble[t] = flag ? 1 : 0;
-  if (tmask) tmask[t] = 0;
-  if (list) {
-    list_append(flag, (int)t, list, lst);
-    band_append(band, (int)t, blist, lst);
-  }
+  prob[t] = p;
+  screen[t] = p >= lo ? 1 : 0;
 }
 
-// KXRCF_Detector on the cells of the band list; flagged ones join the
-// troubled-cell list
-__global__ __launch_bounds__(BS) void k_kxrcf_band(const double* __restrict__ U, int* __restrict__ trouble,
-                                                   int* __restrict__ list, const int* __restrict__ blist, Geo g,
-                                                   double Ck, double hk, DevScal* sc, DevScal* lst) {
+// 1 where the screen marks interior cell (a, b) (0-based); outside the
+// domain the periodic image, or nothing for outflow boundaries
+__device__ __forceinline__ int screen_at(const int* __restrict__ screen, const Geo& g, int a, int b) {
+  if (a < 0) { if (g.bcL != 1) return 0; a += g.Nx; }
+  if (a >= g.Nx) { if (g.bcR != 1) return 0; a -= g.Nx; }
+  if (b < 0) { if (g.bcD != 1) return 0; b += g.Ny; }
+  if (b >= g.Ny) { if (g.bcU != 1) return 0; b -= g.Ny; }
+  return screen[(long long)b * g.Nx + a];
+}
+
+__device__ __forceinline__ bool candidate(const int* __restrict__ screen, const Geo& g, int i, int j) {
+  return screen_at(screen, g, i, j) | screen_at(screen, g, i + 1, j) | screen_at(screen, g, i - 1, j) |
+         screen_at(screen, g, i, j + 1) | screen_at(screen, g, i, j - 1);
+}
+
+// candidates = screen dilated by one cell; every cell's TVB marks are
+// cleared for the step
+__global__ __launch_bounds__(BS) void k_dilate(const int* __restrict__ screen, int* __restrict__ trouble,
+                                               int* __restrict__ tmask, int* __restrict__ blist, Geo g,
+                                               DevScal* lst) {
+  const long long t = xcd_block() * BS + threadIdx.x;
+  if (t >= (long long)g.Nx * g.Ny) return;
+  const bool cand = candidate(screen, g, (int)(t % g.Nx), (int)(t / g.Nx));
+  trouble[t] = 0;
+  tmask[t] = 0;
+  band_append(cand, (int)t, blist, lst);
+}
+
+__global__ void k_reset_nlist(DevScal* lst) {
+  if (threadIdx.x == 0) lst->nlist = 0u;
+}
+
+// KXRCF_Detector on the candidates; flagged ones join the troubled-cell list
+__global__ __launch_bounds__(BS) void k_kxrcf_cands(const double* __restrict__ U, int* __restrict__ trouble,
+                                                    int* __restrict__ tmask, int* __restrict__ list,
+                                                    const int* __restrict__ blist, Geo g, double Ck, double hk,
+                                                    DevScal* sc, DevScal* lst) {
   const unsigned int nb = lst->nband;
   for (unsigned int k0 = blockIdx.x * BS; k0 < nb; k0 += gridDim.x * BS) {
     const unsigned int k = k0 + threadIdx.x;
@@ -263,14 +290,16 @@ __global__ __launch_bounds__(BS) void k_kxrcf_band(const double* __restrict__ U,
       t = blist[k];
       const int i = 1 + t % g.Nx, j = 1 + t / g.Nx;
       flag = kxrcf_cell(U, g, i, j, Ck, hk);
-      if (flag) trouble[t] = 1;
+      trouble[t] = flag ? 1 : 0;
+      tmask[t] = 0;
     }
+    if (sc) count_flags(flag, sc);
     list_append(flag, t, list, lst);
   }
 }
 
-__global__ void k_list_reset_mlp(DevScal* lst) {
-  if (threadIdx.x == 0) { lst->nlist = 0u; lst->nband = 0u; }
+__global__ void k_reset_nband(DevScal* lst) {
+  if (threadIdx.x == 0) lst->nband = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -408,16 +437,17 @@ __global__ __launch_bounds__(BS) void k_rows(long long* __restrict__ rows, int B
   rows[r] = (long long)(mix64(seed + 0x9e3779b97f4a7c15ull * (unsigned long long)(r + 1)) % (unsigned long long)N);
 }
 
-// indicator statistics over the interior cells (see mhd2d_indicator_stats)
-__global__ __launch_bounds__(BS) void k_istats(const int* __restrict__ kx, const int* __restrict__ tm,
-                                               const float* __restrict__ prob, float lo, float hi, long long n,
+// indicator statistics over the interior cells (see mhd2d_indicator_stats);
+// the decision on a candidate is KXRCF's
+__global__ __launch_bounds__(BS) void k_istats(const int* __restrict__ screen, const int* __restrict__ kx,
+                                               const int* __restrict__ tm, Geo g,
                                                unsigned long long* __restrict__ out) {
   const long long t = (long long)blockIdx.x * BS + threadIdx.x;
-  if (t >= n) return;
+  if (t >= (long long)g.Nx * g.Ny) return;
+  const bool cand = candidate(screen, g, (int)(t % g.Nx), (int)(t / g.Nx));
   const bool k = kx[t] != 0, c = tm[t] != 0, lab = k && c;
-  const bool band = prob[t] >= lo && prob[t] < hi;
-  const bool d = prob[t] >= hi || (band && k);     // the limiter pass's decision
-  const bool v[8] = {true, lab, k, d, band, lab && !d, d && !lab, d && !k && c};
+  const bool d = cand && k;
+  const bool v[8] = {true, lab, k, d, cand, lab && !cand, cand && !lab, d && c && !k};
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const unsigned long long b = __ballot(v[q]);
@@ -460,23 +490,39 @@ int mlp_upload(mhd2d_ctx* ctx, const float* params) {
   return 0;
 }
 
-int launch_mlp_detect(mhd2d_ctx* ctx, const double* in, DevScal* sc, bool list, float* prob) {
+static inline unsigned nb_blocks(long long n) { return (unsigned)((n + BS - 1) / BS); }
+
+static int launch_screen_only(mhd2d_ctx* ctx, const double* in) {
   const Geo& g = ctx->g;
   if (!ctx->mlpw) return fail(ctx, -6, "indicator MLP: no weights (mhd2d_mlp_set_weights)");
   const long long ncell = (long long)g.Nx * g.Ny;
-  if (list) hipLaunchKernelGGL(k_list_reset_mlp, dim3(1), dim3(64), 0, ctx->stream, ctx->sc);
-  hipLaunchKernelGGL(k_mlp_detect, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, ctx->stream, in,
-                     ctx->trouble, ctx->tmask, list ? ctx->tlist : nullptr, ctx->blist, prob,
-                     (const MlpDev*)ctx->mlpw, g, (float)log10(ctx->cfg.M * g.hx * g.hy),
-                     (float)(0.5 * log10(sqrt(g.hx * g.hy))), (float)ctx->cfg.mlp_lo, (float)ctx->cfg.mlp_hi,
-                     ctx->sc);
-  if (list) {
-    const unsigned nb = std::min<unsigned>((unsigned)((ncell + BS - 1) / BS), 1024u);
-    hipLaunchKernelGGL(k_kxrcf_band, dim3(nb), dim3(BS), 0, ctx->stream, in, ctx->trouble, ctx->tlist,
-                       (const int*)ctx->blist, g, ctx->cfg.Ck, kx_hk(g), sc, ctx->sc);
-  }
+  hipLaunchKernelGGL(k_mlp_screen, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, ctx->stream, in,
+                     ctx->screen, ctx->prob, (const MlpDev*)ctx->mlpw, g, (float)log10(ctx->cfg.M * g.hx * g.hy),
+                     (float)(0.5 * log10(sqrt(g.hx * g.hy))), (float)ctx->cfg.mlp_lo);
   LAUNCHCHK(ctx);
   return 0;
+}
+
+int launch_mlp_screen(mhd2d_ctx* ctx, const double* in) {
+  const Geo& g = ctx->g;
+  int rc = launch_screen_only(ctx, in);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_reset_nband, dim3(1), dim3(64), 0, ctx->stream, ctx->sc);
+  hipLaunchKernelGGL(k_dilate, dim3(nb_blocks((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream,
+                     (const int*)ctx->screen, ctx->trouble, ctx->tmask, ctx->blist, g, ctx->sc);
+  LAUNCHCHK(ctx);
+  return 0;
+}
+
+int launch_candidates_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc) {
+  const Geo& g = ctx->g;
+  const long long ncell = (long long)g.Nx * g.Ny;
+  const unsigned nb = std::min<unsigned>(nb_blocks(ncell), 1024u);
+  hipLaunchKernelGGL(k_reset_nlist, dim3(1), dim3(64), 0, ctx->stream, ctx->sc);
+  hipLaunchKernelGGL(k_kxrcf_cands, dim3(nb), dim3(BS), 0, ctx->stream, in, ctx->trouble, ctx->tmask, ctx->tlist,
+                     (const int*)ctx->blist, g, ctx->cfg.Ck, kx_hk(g), sc, ctx->sc);
+  LAUNCHCHK(ctx);
+  return launch_tvb_list(ctx, in);
 }
 
 }  // namespace mhd2d_impl
@@ -558,7 +604,7 @@ int mhd2d_mlp_set_weights(mhd2d_ctx* ctx, const float* params, int32_t n) {
 int mhd2d_mlp_probability(mhd2d_ctx* ctx, float* host_p) {
   if (!ctx) return -1;
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  int rc = launch_mlp_detect(ctx, ctx->S, nullptr, false, ctx->prob);
+  int rc = launch_screen_only(ctx, ctx->S);
   if (rc) return rc;
   const long long ncell = (long long)ctx->g.Nx * ctx->g.Ny;
   if (host_p) HIPCHK(ctx, hipMemcpyAsync(host_p, ctx->prob, ncell * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
@@ -574,13 +620,12 @@ int mhd2d_indicator_stats(mhd2d_ctx* ctx, int64_t* out) {
   unsigned long long* cnt = nullptr;
   HIPCHK(ctx, hipMalloc(&kx, ncell * sizeof(int)));
   HIPCHK(ctx, hipMalloc(&cnt, 8 * sizeof(unsigned long long)));
-  int rc = launch_mlp_detect(ctx, ctx->S, nullptr, false, ctx->prob);
+  int rc = launch_screen_only(ctx, ctx->S);
   if (!rc) rc = launch_tvb_all(ctx, ctx->S, kx);   // KXRCF flags into kx, TVB change bits of every cell into tmask
   if (!rc) {
     (void)hipMemsetAsync(cnt, 0, 8 * sizeof(unsigned long long), ctx->stream);
-    hipLaunchKernelGGL(k_istats, dim3(nb_of(ncell)), dim3(BS), 0, ctx->stream, (const int*)kx,
-                       (const int*)ctx->tmask, (const float*)ctx->prob, (float)ctx->cfg.mlp_lo,
-                       (float)ctx->cfg.mlp_hi, ncell, cnt);
+    hipLaunchKernelGGL(k_istats, dim3(nb_of(ncell)), dim3(BS), 0, ctx->stream, (const int*)ctx->screen,
+                       (const int*)kx, (const int*)ctx->tmask, ctx->g, cnt);
     unsigned long long h[8];
     (void)hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
     hipError_t e = hipStreamSynchronize(ctx->stream);

@@ -110,12 +110,15 @@ int mhd2d_profile_get(mhd2d_ctx *ctx, double *ms, int64_t *launches, int32_t ncl
 
 /* ---- learned troubled-cell indicator (no counterpart in the reference) ----
    A 60-64-64-1 MLP (GELU hidden layers, sigmoid output) over a cell's and its
-   four neighbours' normalised modal coefficients (density and total energy,
-   6 modes each).  With indicator = MHD2D_IND_MLP the limiter pass applies TVB
-   where p >= mlp_hi, skips cells with p < mlp_lo and asks KXRCF_Detector in
-   between.  Parameters are one FP32 vector of MHD2D_MLP_NPARAM values:
-   W1[60][64], b1[64], W2[64][64], b2[64], w3[64], b3 (W rows = inputs).
-   Inference runs in BF16 on the matrix cores with FP32 accumulation. */
+   four neighbours' normalised modal coefficients gives p = P(the plain method
+   limits the cell).  With indicator = MHD2D_IND_MLP the limiter pass hands
+   the cells with p >= mlp_lo, dilated by one cell, to KXRCF_Detector and
+   skips the rest; inside mhd2d_step the network runs once per step, on the
+   first stage's limiter input, and its candidates serve all three stages.
+   mlp_hi is reserved.  Parameters are one FP32 vector of MHD2D_MLP_NPARAM
+   values: W1[60][64], b1[64], W2[64][64], b2[64], w3[64], b3 (W rows =
+   inputs).  Inference runs in BF16 on the matrix cores with FP32
+   accumulation. */
 #define MHD2D_MLP_NPARAM 8129
 #define MHD2D_MLP_NIN 60
 int mhd2d_mlp_set_weights(mhd2d_ctx *ctx, const float *params, int32_t n);
@@ -123,10 +126,10 @@ int mhd2d_mlp_set_weights(mhd2d_ctx *ctx, const float *params, int32_t n);
    host_p is (Nx, Ny) column-major (may be NULL) */
 int mhd2d_mlp_probability(mhd2d_ctx *ctx, float *host_p);
 /* indicator statistics on the current state, out[8] = { cells, label
-   (KXRCF flags AND TVB changes), KXRCF flags, MLP decisions (TVB applied),
-   cells in the confidence band, missed (label and not applied), over-flagged
-   (applied and not label), over-flagged that TVB changes (applied, not
-   KXRCF, TVB changes) } */
+   (KXRCF flags AND TVB changes), KXRCF flags, decisions (candidates KXRCF
+   flags: TVB applied), candidates (cells handed to KXRCF), missed (label and
+   not a candidate), over (candidate and not label), TVB changes the plain
+   method would not make (0 by construction) } */
 int mhd2d_indicator_stats(mhd2d_ctx *ctx, int64_t *out);
 
 /* FP32 training of the indicator on the GPU.  A trainer owns a device

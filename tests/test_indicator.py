@@ -9,7 +9,7 @@
 - BF16-MFMA inference == oracle forward within BF16 rounding, for every cell
 - the trained network shipped with the package (indicator_weights.npz):
   missed-shock and over-flag rates on held-out rotor and Riemann states,
-  where KXRCF is evaluated only inside the confidence band; on the smooth
+  where KXRCF is evaluated only on the screen's candidates; on the smooth
   benchmark flow (another seed) the mixed form with the learned indicator
   stays within 1e-5 (density, rel L2) of the plain FP64 method
 """
@@ -131,9 +131,9 @@ MISS_BUDGET = {"rotor": 0.02, "riemann": 0.05}
 
 @pytest.mark.parametrize("kind", ["rotor", "riemann"])
 def test_trained_indicator_heldout(pkg, trained, kind):
-    """held-out seeded IC at 128^2: the classical detector runs only inside
-    the band, missed shocks stay rare, and the indicator-gated run stays
-    close to the plain method's"""
+    """held-out IC at the training resolution (256^2) of these flows: the
+    classical detector runs only on the screen's candidates, missed shocks
+    stay rare, and the indicator-gated run stays close to the plain method's"""
     import importlib
     ind = importlib.import_module("freljord_cfd_arsenal_amd.indicator")
     params, lo, hi = trained
@@ -146,24 +146,24 @@ def test_trained_indicator_heldout(pkg, trained, kind):
     runs = {}
     cfl = 0.15
     for name, indicator in (("plain", pkg.IND_KXRCF), ("mlp", pkg.IND_MLP)):
-        s = pkg.Solver(128, 128, flux_type=pkg.HLLD, bc=bc, domain=dom, M=M, CFL=cfl, pp=1, indicator=indicator,
+        s = pkg.Solver(256, 256, flux_type=pkg.HLLD, bc=bc, domain=dom, M=M, CFL=cfl, pp=1, indicator=indicator,
                        Ck=1.0, mlp_band=(lo, hi))
         s.mlp_set_weights(params)
         s.init_data(kind_, extra)
         s.rk3_prologue()
-        for _ in range(30):
+        for _ in range(20):
             s.step()
         runs[name] = s
     st = runs["plain"].indicator_stats()
     print(f"[indicator] {kind}: {st}")
     # the classical detector is asked for a minority of the cells, and the
     # learned decisions rarely differ from the plain method's where it matters
-    assert st["band"] <= 0.25 * st["cells"]
+    assert st["candidates"] <= 0.5 * st["cells"]
     assert st["missed"] <= max(2, MISS_BUDGET[kind] * st["label"])
-    assert st["over_changed"] <= max(2, 0.05 * st["label"])
+    assert st["over_changed"] == 0
     a, b = runs["mlp"].download()[0], runs["plain"].download()[0]
     err = rel_l2(a[1:-1, 1:-1, :, 0], b[1:-1, 1:-1, :, 0])
-    print(f"[indicator] {kind}: density rel L2 mlp vs plain after 30 steps: {err:.3e}")
+    print(f"[indicator] {kind}: density rel L2 mlp vs plain after 20 steps: {err:.3e}")
     # shock-dominated flows: a single cell limited differently moves its
     # neighbourhood by O(jump) and the flow amplifies it; the 1e-5 budget is
     # the smooth benchmark flow's (test_trained_indicator_ot)
@@ -186,6 +186,8 @@ def test_trained_indicator_ot(pkg, trained):
         runs[name] = s
     st = runs["plain"].indicator_stats()
     print(f"[indicator] ot: {st}")
+    # the smooth flow: the network screens out nearly every cell
+    assert st["candidates"] <= 0.05 * st["cells"] and st["missed"] <= max(2, 0.05 * st["label"])
     a, b = runs["mlp"].download()[0], runs["plain"].download()[0]
     err = rel_l2(a[1:-1, 1:-1, :, 0], b[1:-1, 1:-1, :, 0])
     print(f"[indicator] ot: density rel L2 mixed+mlp vs plain FP64 after 40 steps: {err:.3e}")
