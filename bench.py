@@ -31,28 +31,31 @@ HBM_PEAK_GBS = 8000.0
 DOF_PER_CELL = 48
 
 # algorithmic bytes per interior cell for one launch of each kernel class.
-# State: 52 FP64 planes (416 B/cell) in both forms; face B: 6 FP64 planes
-# (48 B/cell).  Face fluxes (40 values per face), face EMFs (5 per face)
+# State: 52 FP64 planes (416 B/cell) in the plain form, 296 B/cell in the
+# mixed storage; face B: 6 FP64 planes (48 B/cell).  Face fluxes (40 values per face), face EMFs (5 per face)
 # are FP64 in the plain form and FP32 in the mixed form (bytes B below).
 # One x face and one y face belong to each cell.
 def alg_bytes(form):
     B = 8 if form == "fp64" else 4
+    # one state: 52 FP64 planes; mixed storage: 22 FP64 planes (cell
+    # averages, magnetic modes) + 30 FP32 planes (higher modes of the rest)
+    S = 416 if form == "fp64" else 22 * 8 + 30 * 4
     return {
-        "xface": 416 + 45 * B,          # state in; 40 fluxes + 5 face EMFs out
-        "yface": 416 + 45 * B,
-        "vertex": 416 + 8,              # state in; one FP64 vertex EMF out
+        "xface": S + 45 * B,          # state in; 40 fluxes + 5 face EMFs out
+        "yface": S + 45 * B,
+        "vertex": S + 8,              # state in; one FP64 vertex EMF out
         # state + x/y fluxes in, stage value out; the RK inputs average
         # (0 + 2 + 2) / 3 state reads (stages 2 and 3 read U0 and the
         # unlimited previous stage value)
-        "cell": 416 + 2 * 40 * B + 416 + 4 * 416 / 3.0,
+        "cell": S + 2 * 40 * B + S + 4 * S / 3.0,
         "ct": 2 * 5 * B + 8 + 48 + 48 + 4 * 48 / 3.0,   # face + vertex EMFs, faces in/out, RK inputs
         # limiter pass: the detector/TVB kernel reads the state (KXRCF
         # traces), the apply pass reads state + faces and writes the state
-        "limit": 416 + 416 + 48 + 416 + 8,
+        "limit": S + S + 48 + S + 8,
         "setbc": 0.0,
         "dt": 64,                       # 8 cell averages
-        "kxrcf": 416,
-        "diag": 416,
+        "kxrcf": S,
+        "diag": S,
     }
 
 

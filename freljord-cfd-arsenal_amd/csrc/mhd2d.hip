@@ -30,8 +30,8 @@ namespace {
 
 // set_bc.f90:5-27 over the halo ring: periodic (1) or outflow (2); the
 // x-then-y order of the reference makes the corners (map_x(i), map_y(j)).
-__global__ __launch_bounds__(BS) void k_setbc(double* __restrict__ U, Geo g, int nplanes,
-                                              const DevScal* __restrict__ sc) {
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_setbc(SWr<MIX> U, Geo g, int nplanes, const DevScal* __restrict__ sc) {
   const long long t = (long long)blockIdx.x * BS + threadIdx.x;
   const int W = g.Nx + 2, H = g.Ny + 2;
   const long long ring = 2LL * W + 2LL * g.Ny;
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(BS) void k_setbc(double* __restrict__ U, Geo g, int
   };
   const int si = mapx(i), sj = mapy(j);
   if (si == i && sj == j) return;
-  U[p * g.plane + cix(g, i, j)] = U[p * g.plane + cix(g, si, sj)];
+  U.set(p, cix(g, i, j), U(p, cix(g, si, sj)));
 }
 
 // ---------------------------------------------------------------------------
@@ -87,14 +87,15 @@ __global__ void k_scal_reset(DevScal* sc, int which) {
 }
 
 // calculate_dt.f90:12-24: max |u_n +- c_f| at the cell averages
-__global__ __launch_bounds__(BS) void k_dt(const double* __restrict__ U, Geo g, DevScal* sc) {
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_dt(SRd<MIX> U, Geo g, DevScal* sc) {
   const long long t = (long long)blockIdx.x * BS + threadIdx.x;
   double ax = 0.0, ay = 0.0;
   if (t < (long long)g.Nx * g.Ny) {
     const int i = 1 + (int)(t % g.Nx), j = 1 + (int)(t / g.Nx);
     double u[8];
 #pragma unroll
-    for (int n = 0; n < 8; n++) u[n] = U[PL(0, n) * g.plane + cix(g, i, j)];
+    for (int n = 0; n < 8; n++) u[n] = U(PL(0, n), cix(g, i, j));
     double a1, a2;
     eigenvalueMm<double, 1>(u, a1, a2);
     const double mx = fmax(fabs(a1), fabs(a2));
@@ -136,7 +137,8 @@ __global__ void k_dt_final(DevScal* sc, double hx, double hy, int advance) {
 
 // calculate_umax.f90, calculate_pmin.f90 and per-block partial sums of
 // calculate_totaldiv.f90 (summed in a fixed order by k_div_final)
-__global__ __launch_bounds__(BS) void k_diag(const double* __restrict__ U, Geo g, DevScal* sc, double* partial) {
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_diag(SRd<MIX> U, Geo g, DevScal* sc, double* partial) {
   const long long t = (long long)blockIdx.x * BS + threadIdx.x;
   double um = 0.0, pm = 1000000.0, dv = 0.0;
   if (t < (long long)g.Nx * g.Ny) {
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(BS) void k_diag(const double* __restrict__ U, Geo g
     const long long c = cix(g, i, j), pl = g.plane;
     double m[NPL];
 #pragma unroll
-    for (int p = 0; p < NPL; p++) m[p] = U[p * pl + c];
+    for (int p = 0; p < NPL; p++) m[p] = U(p, c);
     um = fabs(m[PL(0, 0)]);
     const double ihx1 = 1.0 / g.hx1, ihy1 = 1.0 / g.hy1;
 #pragma unroll
@@ -258,15 +260,16 @@ __device__ void ic_eval(const ICPar& c, double x, double y, double U[8]) {
   U[7] = B3;
 }
 
-__global__ __launch_bounds__(BS) void k_init_cells(double* __restrict__ S, Geo g, double xa, double ya,
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_init_cells(SWr<MIX> S, Geo g, double xa, double ya,
                                                    const ICPar* __restrict__ icp) {
   const long long t = (long long)blockIdx.x * BS + threadIdx.x;
   const int W = g.Nx + 2;
   if (t >= (long long)W * (g.Ny + 2)) return;
   const int i = (int)(t % W), j = (int)(t / W);
-  const long long c = cix(g, i, j), pl = g.plane;
+  const long long c = cix(g, i, j);
   if (!(i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny)) {
-    for (int p = 0; p < NPL; p++) S[p * pl + c] = 0.0;
+    for (int p = 0; p < NPL; p++) S.set(p, c, 0.0);
     return;
   }
   const ICPar& ic = *icp;
@@ -287,7 +290,7 @@ __global__ __launch_bounds__(BS) void k_init_cells(double* __restrict__ S, Geo g
     for (int b = 0; b < 10; b++) {
       const int p = PL(b, n);
       if (p < 0) continue;
-      S[p * pl + c] = b < 6 ? (0.25 * acc[n][b]) / mmass(b) : 0.0;   // init_data.f90:43 leaves modes 7-10 at 0
+      S.set(p, c, b < 6 ? (0.25 * acc[n][b]) / mmass(b) : 0.0);   // init_data.f90:43 leaves modes 7-10 at 0
     }
 }
 
@@ -325,6 +328,17 @@ __global__ __launch_bounds__(BS) void k_init_faces(double* __restrict__ F, Geo g
   }
 }
 
+// mixed-form storage <-> 52 FP64 planes (the FP32 planes round to nearest)
+template <bool TO_MIX>
+__global__ __launch_bounds__(BS) void k_pack(double* __restrict__ full, SWr<true> v) {
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  if (t >= (long long)NPL * v.pl) return;
+  const int p = (int)(t / v.pl);
+  const long long c = t % v.pl;
+  if (TO_MIX) v.set(p, c, full[t]);
+  else full[t] = v(p, c);
+}
+
 }  // namespace
 
 namespace {
@@ -333,7 +347,29 @@ int launch_setbc(mhd2d_ctx* ctx, double* U, const DevScal* sc) {
   const Geo& g = ctx->g;
   const long long ring = 2LL * (g.Nx + 2) + 2LL * g.Ny;
   Timed tm(ctx, KC_SETBC);
-  hipLaunchKernelGGL(k_setbc, dim3(nblk(ring * NPL)), dim3(BS), 0, ctx->stream, U, g, NPL, sc);
+  return with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_setbc<MIX>), dim3(nblk(ring * NPL)), dim3(BS), 0, ctx->stream, SWr<MIX>{U, g.plane}, g, NPL,
+                       sc);
+    LAUNCHCHK(ctx);
+    return 0;
+  });
+}
+
+// the mixed form's upload/download staging buffer (52 FP64 planes, zeroed once)
+int mix_stage(mhd2d_ctx* ctx) {
+  if (ctx->stage) return 0;
+  const size_t n = (size_t)NPL * ctx->g.plane * sizeof(double);
+  HIPCHK(ctx, hipMalloc((void**)&ctx->stage, n));
+  HIPCHK(ctx, hipMemsetAsync(ctx->stage, 0, n, ctx->stream));
+  return 0;
+}
+
+int launch_pack(mhd2d_ctx* ctx, bool to_mix, double* mixbuf) {
+  const long long n = (long long)NPL * ctx->g.plane;
+  const SWr<true> v{mixbuf, ctx->g.plane};
+  if (to_mix) hipLaunchKernelGGL(k_pack<true>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, ctx->stage, v);
+  else hipLaunchKernelGGL(k_pack<false>, dim3(nblk(n)), dim3(BS), 0, ctx->stream, ctx->stage, v);
   LAUNCHCHK(ctx);
   return 0;
 }
@@ -381,7 +417,7 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
   g.hy1 = 0.5 * g.hy;
   hipError_t e = hipGetDevice(&ctx->device);
   if (e != hipSuccess) { free(ctx); return -2; }
-  const size_t sbytes = (size_t)NPL * g.plane * sizeof(double);
+  const size_t sbytes = (size_t)state_bytes(cfg->precision == MHD2D_MIXED, g.plane);
   const size_t fbytes = (size_t)6 * g.plane * sizeof(double);
   const size_t xbytes = (size_t)40 * g.plane * sizeof(double);
   const size_t ebytes = (size_t)5 * g.plane * sizeof(double);
@@ -435,7 +471,7 @@ int mhd2d_destroy(mhd2d_ctx* ctx) {
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   ctx->prof_ev.~vector();
   ctx->ev_pool.~vector();
-  void* ps[] = {ctx->S, ctx->U0, ctx->UI, ctx->UII, ctx->T, ctx->du, ctx->F, ctx->F0, ctx->FI, ctx->FII,
+  void* ps[] = {ctx->stage, ctx->S, ctx->U0, ctx->UI, ctx->UII, ctx->T, ctx->du, ctx->F, ctx->F0, ctx->FI, ctx->FII,
                 ctx->Ft, ctx->dEz, ctx->FX, ctx->FY, ctx->EZX, ctx->EZY, ctx->EZV, ctx->trouble, ctx->tmask,
                 ctx->tlist, ctx->blist, ctx->screen, ctx->trec, ctx->prob, ctx->mlpw, ctx->sc, ctx->partial, ctx->icp};
   for (void* p : ps)
@@ -458,15 +494,19 @@ int mhd2d_upload(mhd2d_ctx* ctx, const double* uh, const double* Bx, const doubl
   if (rc) return rc;
   const Geo& g = ctx->g;
   const size_t W = g.Nx + 2, H = g.Ny + 2, P = g.P;
+  const bool mix = ctx->cfg.precision == MHD2D_MIXED;
+  if (uh && mix && (rc = mix_stage(ctx))) return rc;
+  double* dst = mix ? ctx->stage : ctx->S;
   if (uh) {
     for (int n = 0; n < 8; n++)
       for (int b = 0; b < 10; b++) {
         const int p = PL(b, n);
         if (p < 0) continue;
         const double* src = uh + W * H * (b + 10 * n);
-        HIPCHK(ctx, hipMemcpy2DAsync(ctx->S + p * g.plane + OFF, P * 8, src, W * 8, W * 8, H, hipMemcpyHostToDevice,
+        HIPCHK(ctx, hipMemcpy2DAsync(dst + p * g.plane + OFF, P * 8, src, W * 8, W * 8, H, hipMemcpyHostToDevice,
                                      ctx->stream));
       }
+    if (mix && (rc = launch_pack(ctx, true, ctx->S))) return rc;
   }
   if (Bx)
     for (int k = 0; k < 3; k++)
@@ -481,9 +521,14 @@ int mhd2d_upload(mhd2d_ctx* ctx, const double* uh, const double* Bx, const doubl
   return 0;
 }
 
-static int download_planes(mhd2d_ctx* ctx, const double* dev, double* uh) {
+static int download_planes(mhd2d_ctx* ctx, double* dev, double* uh) {
   const Geo& g = ctx->g;
   const size_t W = g.Nx + 2, H = g.Ny + 2, P = g.P;
+  if (ctx->cfg.precision == MHD2D_MIXED) {
+    int rc = mix_stage(ctx);
+    if (rc || (rc = launch_pack(ctx, false, dev))) return rc;
+    dev = ctx->stage;
+  }
   for (int n = 0; n < 8; n++)
     for (int b = 0; b < 10; b++) {
       double* dst = uh + W * H * (b + 10 * n);
@@ -579,8 +624,12 @@ int mhd2d_init_data(mhd2d_ctx* ctx, int32_t ic, const double* extra, int32_t nex
   HIPCHK(ctx, hipMemcpyAsync(ctx->icp, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
   const long long nc = (long long)(g.Nx + 2) * (g.Ny + 2);
   const long long nf = (long long)(g.Nx + 1) * g.Ny + (long long)g.Nx * (g.Ny + 1);
-  hipLaunchKernelGGL(k_init_cells, dim3(nblk(nc)), dim3(BS), 0, ctx->stream, ctx->S, g, ctx->cfg.xa, ctx->cfg.ya,
-                     (const ICPar*)ctx->icp);
+  rc = with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_init_cells<MIX>), dim3(nblk(nc)), dim3(BS), 0, ctx->stream, SWr<MIX>{ctx->S, g.plane}, g,
+                       ctx->cfg.xa, ctx->cfg.ya, (const ICPar*)ctx->icp);
+    return 0;
+  });
   hipLaunchKernelGGL(k_init_faces, dim3(nblk(nf)), dim3(BS), 0, ctx->stream, ctx->F, g, ctx->cfg.xa, ctx->cfg.ya,
                      (const ICPar*)ctx->icp);
   LAUNCHCHK(ctx);
@@ -616,11 +665,13 @@ int mhd2d_lhez(mhd2d_ctx* ctx) {
 int mhd2d_rk_stage(mhd2d_ctx* ctx, int32_t stage, double dt) {
   int rc = ctx_check(ctx);
   if (rc) return rc;
-  const long long ns = (long long)NPL * ctx->g.plane, nf = 6LL * ctx->g.plane;
-  const size_t bs = ns * sizeof(double), bf = nf * sizeof(double);
+  const long long nf = 6LL * ctx->g.plane;
+  const size_t bs = (size_t)state_bytes(ctx->cfg.precision == MHD2D_MIXED, ctx->g.plane), bf = nf * sizeof(double);
+  const long long ns = -1;     // state buffers: launch_axpy_state
   hipStream_t st = ctx->stream;
   auto ax = [&](double* out, const double* X, double a, const double* Y, double b, const double* Z, double d, long long n) {
-    (void)launch_axpy(ctx, out, X, a, Y, b, Z, d, n);
+    if (n < 0) (void)launch_axpy_state(ctx, out, X, a, Y, b, Z, d);
+    else (void)launch_axpy(ctx, out, X, a, Y, b, Z, d, n);
   };
   if (stage == 1) {
     ax(ctx->UI, nullptr, 0, ctx->S, 1, ctx->du, dt, ns);
@@ -694,7 +745,12 @@ static int launch_dt(mhd2d_ctx* ctx, int advance) {
   const Geo& g = ctx->g;
   Timed tm(ctx, KC_DT);
   hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, 0);
-  hipLaunchKernelGGL(k_dt, dim3(ctx->nblk_cells), dim3(BS), 0, ctx->stream, ctx->S, g, ctx->sc);
+  with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_dt<MIX>), dim3(ctx->nblk_cells), dim3(BS), 0, ctx->stream, SRd<MIX>{ctx->S, g.plane}, g,
+                       ctx->sc);
+    return 0;
+  });
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, g.hx, g.hy, advance);
   LAUNCHCHK(ctx);
   return 0;
@@ -715,7 +771,12 @@ static int launch_diag(mhd2d_ctx* ctx) {
   const Geo& g = ctx->g;
   Timed tm(ctx, KC_DIAG);
   hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, 1);
-  hipLaunchKernelGGL(k_diag, dim3(ctx->nblk_cells), dim3(BS), 0, ctx->stream, ctx->S, g, ctx->sc, ctx->partial);
+  with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_diag<MIX>), dim3(ctx->nblk_cells), dim3(BS), 0, ctx->stream, SRd<MIX>{ctx->S, g.plane}, g,
+                       ctx->sc, ctx->partial);
+    return 0;
+  });
   hipLaunchKernelGGL(k_div_final, dim3(1), dim3(BS), 0, ctx->stream, ctx->sc, (const double*)ctx->partial,
                      ctx->nblk_cells, g.hx1, g.hy1);
   LAUNCHCHK(ctx);

@@ -27,10 +27,10 @@ namespace {
 // in registers next to the 52 accumulators would spill, and re-reading them
 // from global memory misses L2 at this occupancy (measured: 2.5x the
 // algorithmic traffic).  In the plain form the epilogue takes S from LDS too.
-template <int MODE, typename T, int CBS>
-__global__ __launch_bounds__(CBS) void k_cell(const double* __restrict__ S, const T* __restrict__ FX,
-                                              const T* __restrict__ FY, const double* __restrict__ A,
-                                              const double* __restrict__ U0, double* __restrict__ out, Geo g,
+template <int MODE, typename T, int CBS, bool MIX = (sizeof(T) == 4)>
+__global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ FX,
+                                              const T* __restrict__ FY, SRd<MIX> A,
+                                              SRd<MIX> U0, SWr<MIX> out, Geo g,
                                               const DevScal* __restrict__ sc, double dt_arg) {
   __shared__ T sm[NPL * CBS];
   const int tid = threadIdx.x;
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(CBS) void k_cell(const double* __restrict__ S, cons
   for (int p = 0; p < NPL; p++) acc[p] = T(0);
   if (interior) {
 #pragma unroll
-    for (int p = 0; p < NPL; p++) sm[p * CBS + tid] = T(S[p * pl + c]);
+    for (int p = 0; p < NPL; p++) sm[p * CBS + tid] = T(S(p, c));
     const T ihx1 = T(1.0 / g.hx1), ihy1 = T(1.0 / g.hy1);
 #pragma unroll
     for (int j1 = 0; j1 < 5; j1++) {
@@ -128,18 +128,26 @@ __global__ __launch_bounds__(CBS) void k_cell(const double* __restrict__ S, cons
     }
   }
   // epilogue: mass inverse + RK combination, in double
-#pragma unroll
-  for (int p = 0; p < NPL; p++) {
-    const long long o = p * pl + c;
+  auto plane = [&](auto P) {
+    const int p = P;
     const bool avg = p < 48 && (p % 6) == 0;
-    int b = p < 48 ? p % 6 : (p == 48 ? 6 : p == 49 ? 8 : p == 50 ? 7 : 9);
+    const int b = p < 48 ? p % 6 : (p == 48 ? 6 : p == 49 ? 8 : p == 50 ? 7 : 9);
     const double du = avg ? acc0[p / 6] : (double)acc[p] * (1.0 / mmass(b));
     double r;
     if (MODE == 0) r = du;
-    else if (MODE == 1) r = (sizeof(T) == 8 && interior ? (double)sm[p * CBS + tid] : S[o]) + dt * du;
-    else if (MODE == 2) r = (3.0 / 4.0) * U0[o] + (1.0 / 4.0) * (A[o] + dt * du);
-    else r = (1.0 / 3.0) * U0[o] + (2.0 / 3.0) * (A[o] + dt * du);
-    out[o] = r;
+    // the LDS copy is exact for the planes stored in the quadrature type
+    else if (MODE == 1) r = (interior && (sizeof(T) == 8 || !pl_f64(p)) ? (double)sm[p * CBS + tid] : S(p, c)) + dt * du;
+    else if (MODE == 2) r = (3.0 / 4.0) * U0(p, c) + (1.0 / 4.0) * (A(p, c) + dt * du);
+    else r = (1.0 / 3.0) * U0(p, c) + (2.0 / 3.0) * (A(p, c) + dt * du);
+    out.set(p, c, r);
+  };
+  // the mixed storage map needs the plane as a compile-time constant (a
+  // partially unrolled loop would index acc[] dynamically, i.e. in scratch)
+  if constexpr (MIX) {
+    static_for<0, NPL>(plane);
+  } else {
+#pragma unroll
+    for (int p = 0; p < NPL; p++) plane(p);
   }
 }
 
@@ -208,6 +216,16 @@ __global__ __launch_bounds__(BS) void k_ct(const T* __restrict__ EZX, const T* _
   }
 }
 
+// the FP32 planes of the mixed storage: combined in double, stored in float
+__global__ __launch_bounds__(BS) void k_axpy_f(float* __restrict__ out, const float* __restrict__ X, double a,
+                                               const float* __restrict__ Y, double b, const float* __restrict__ Z,
+                                               double dt, long long n) {
+  const long long k = (long long)blockIdx.x * BS + threadIdx.x;
+  if (k >= n) return;
+  const double y = (double)Y[k] + dt * (double)Z[k];
+  out[k] = (float)(X ? a * (double)X[k] + b * y : y);
+}
+
 // whole-array combinations of RK3.f90 for the per-subroutine API
 // (mhd2d_rk_stage): out = a*X + b*(Y + dt*Z), elementwise
 __global__ __launch_bounds__(BS) void k_axpy(double* __restrict__ out, const double* __restrict__ X, double a,
@@ -230,14 +248,16 @@ void launch_cell_t(mhd2d_ctx* ctx, const double* S, const double* A, const doubl
                    const DevScal* sc, double dt) {
   const Geo& g = ctx->g;
   const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
+  const long long pl = g.plane;
   if (ctx->cfg.precision == MHD2D_MIXED)
     hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
-                       dim3(CBS_F32), 0, ctx->stream, S, (const float*)ctx->FX, (const float*)ctx->FY, A, U0, out, g,
-                       sc, dt);
+                       dim3(CBS_F32), 0, ctx->stream, SRd<true>{S, pl}, (const float*)ctx->FX, (const float*)ctx->FY,
+                       SRd<true>{A, pl}, SRd<true>{U0, pl}, SWr<true>{out, pl}, g, sc, dt);
   else
     hipLaunchKernelGGL((k_cell<MODE, double, CBS_F64>), dim3((unsigned)((n + CBS_F64 - 1) / CBS_F64)),
-                       dim3(CBS_F64), 0, ctx->stream, S, (const double*)ctx->FX, (const double*)ctx->FY, A, U0, out,
-                       g, sc, dt);
+                       dim3(CBS_F64), 0, ctx->stream, SRd<false>{S, pl}, (const double*)ctx->FX,
+                       (const double*)ctx->FY, SRd<false>{A, pl}, SRd<false>{U0, pl}, SWr<false>{out, pl}, g, sc,
+                       dt);
 }
 
 int launch_cell(mhd2d_ctx* ctx, int mode, const double* S, const double* A, const double* U0, double* out,
@@ -282,6 +302,22 @@ int launch_ct(mhd2d_ctx* ctx, int mode, const double* F, const double* A, const 
 int launch_axpy(mhd2d_ctx* ctx, double* out, const double* X, double a, const double* Y, double b,
                 const double* Z, double dt, long long n) {
   hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(BS), 0, ctx->stream, out, X, a, Y, b, Z, dt, n);
+  LAUNCHCHK(ctx);
+  return 0;
+}
+
+// the same over state buffers: in the mixed storage the FP64 planes, then
+// the FP32 planes (combined in double, rounded once)
+int launch_axpy_state(mhd2d_ctx* ctx, double* out, const double* X, double a, const double* Y, double b,
+                      const double* Z, double dt) {
+  const long long pl = ctx->g.plane;
+  if (ctx->cfg.precision != MHD2D_MIXED) return launch_axpy(ctx, out, X, a, Y, b, Z, dt, (long long)NPL * pl);
+  int rc = launch_axpy(ctx, out, X, a, Y, b, Z, dt, (long long)NPL_D * pl);
+  if (rc) return rc;
+  auto f = [&](const double* p) { return p ? reinterpret_cast<const float*>(p + NPL_D * pl) : nullptr; };
+  const long long nf = (long long)NPL_F * pl;
+  hipLaunchKernelGGL(k_axpy_f, dim3(nblk(nf)), dim3(BS), 0, ctx->stream, reinterpret_cast<float*>(out + NPL_D * pl),
+                     f(X), a, f(Y), b, f(Z), dt, nf);
   LAUNCHCHK(ctx);
   return 0;
 }

@@ -19,28 +19,27 @@ __host__ __device__ constexpr double WTR(int side, int b) {
   return (s < 1e-13 && s > -1e-13) ? 0.0 : s;
 }
 
-template <int SIDE>
-__device__ __forceinline__ double face_jump_int(const double* __restrict__ U, long long pl, long long cself,
-                                                long long cnb, int n) {
+template <int SIDE, class SV>
+__device__ __forceinline__ double face_jump_int(const SV& U, long long cself, long long cnb, int n) {
   constexpr int OPP = SIDE == 0 ? 1 : SIDE == 1 ? 0 : SIDE == 2 ? 3 : 2;
   double a = 0.0, b = 0.0;
 #pragma unroll
   for (int bb = 0; bb < 10; bb++) {
     const int p = PL(bb, n);
     if (p < 0) continue;
-    if (WTR(SIDE, bb) != 0.0) a += U[p * pl + cself] * WTR(SIDE, bb);
-    if (WTR(OPP, bb) != 0.0) b += U[p * pl + cnb] * WTR(OPP, bb);
+    if (WTR(SIDE, bb) != 0.0) a += U(p, cself) * WTR(SIDE, bb);
+    if (WTR(OPP, bb) != 0.0) b += U(p, cnb) * WTR(OPP, bb);
   }
   return a - b;
 }
 
 // KXRCF_Detector.f90:55-105 for cell (i, j); hk = (hx hy)^((k+1)/4), k = 2
-__device__ __forceinline__ bool kxrcf_cell(const double* __restrict__ U, const Geo& g, int i, int j, double Ck,
-                                           double hk) {
-  const long long pl = g.plane, c = cix(g, i, j);
+template <class SV>
+__device__ __forceinline__ bool kxrcf_cell(const SV& U, const Geo& g, int i, int j, double Ck, double hk) {
+  const long long c = cix(g, i, j);
   const long long cE = cix(g, i + 1, j), cW = cix(g, i - 1, j), cN = cix(g, i, j + 1), cS = cix(g, i, j - 1);
-  const bool inE = U[PL(0, 1) * pl + cE] < 0.0, inW = U[PL(0, 1) * pl + cW] > 0.0;
-  const bool inN = U[PL(0, 2) * pl + cN] < 0.0, inS = U[PL(0, 2) * pl + cS] > 0.0;
+  const bool inE = U(PL(0, 1), cE) < 0.0, inW = U(PL(0, 1), cW) > 0.0;
+  const bool inN = U(PL(0, 2), cN) < 0.0, inS = U(PL(0, 2), cS) > 0.0;
   double Lij = 1e-12;
   if (inE) Lij += 5 * g.hy;
   if (inW) Lij += 5 * g.hy;
@@ -51,16 +50,16 @@ __device__ __forceinline__ bool kxrcf_cell(const double* __restrict__ U, const G
 #pragma unroll
   for (int n = 0; n < 8; n++) {
     double I = 0.0;
-    if (inE) I += g.hy1 * face_jump_int<0>(U, pl, c, cE, n);
-    if (inW) I += g.hy1 * face_jump_int<1>(U, pl, c, cW, n);
-    if (inN) I += g.hx1 * face_jump_int<2>(U, pl, c, cN, n);
-    if (inS) I += g.hx1 * face_jump_int<3>(U, pl, c, cS, n);
+    if (inE) I += g.hy1 * face_jump_int<0>(U, c, cE, n);
+    if (inW) I += g.hy1 * face_jump_int<1>(U, c, cW, n);
+    if (inN) I += g.hx1 * face_jump_int<2>(U, c, cN, n);
+    if (inS) I += g.hx1 * face_jump_int<3>(U, c, cS, n);
     // max over the 25 Gauss points of |u_n| + 1e-12, row by row in eta:
     // u(x, y_j) = c0 + c1 x + c2 P2(x) + c3 P3(x)
-    const double m0 = U[PL(0, n) * pl + c], m1 = U[PL(1, n) * pl + c], m2 = U[PL(2, n) * pl + c];
-    const double m3 = U[PL(3, n) * pl + c], m4 = U[PL(4, n) * pl + c], m5 = U[PL(5, n) * pl + c];
-    const double e6 = n == 5 ? U[PL(6, 5) * pl + c] : n == 6 ? U[PL(7, 6) * pl + c] : 0.0;
-    const double e8 = n == 5 ? U[PL(8, 5) * pl + c] : n == 6 ? U[PL(9, 6) * pl + c] : 0.0;
+    const double m0 = U(PL(0, n), c), m1 = U(PL(1, n), c), m2 = U(PL(2, n), c);
+    const double m3 = U(PL(3, n), c), m4 = U(PL(4, n), c), m5 = U(PL(5, n), c);
+    const double e6 = n == 5 ? U(PL(6, 5), c) : n == 6 ? U(PL(7, 6), c) : 0.0;
+    const double e8 = n == 5 ? U(PL(8, 5), c) : n == 6 ? U(PL(9, 6), c) : 0.0;
     double uinf = 0.0;
 #pragma unroll
     for (int j1 = 0; j1 < 5; j1++) {

@@ -16,8 +16,8 @@ namespace {
 // get_basis.f90), so the 52 modes of the two cells are read once per face.
 // The plain form keeps the reference's per-point sums (k_xface_pt /
 // k_yface_pt below).
-template <int SIDE, typename T>
-__device__ __forceinline__ void trace_coef(const double* __restrict__ U, long long pl, long long c, T cf[8][4]) {
+template <int SIDE, typename T, class SV>
+__device__ __forceinline__ void trace_coef(const SV& U, long long c, T cf[8][4]) {
 #pragma unroll
   for (int n = 0; n < 8; n++) {
     T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
@@ -25,7 +25,7 @@ __device__ __forceinline__ void trace_coef(const double* __restrict__ U, long lo
     for (int b = 0; b < 10; b++) {
       const int p = PL(b, n);
       if (p < 0) continue;
-      const T m = T(U[p * pl + c]);
+      const T m = T(U(p, c));
       if (trk(SIDE, b, 0) != 0.0) a0 += m * T(trk(SIDE, b, 0));
       if (trk(SIDE, b, 1) != 0.0) a1 += m * T(trk(SIDE, b, 1));
       if (trk(SIDE, b, 2) != 0.0) a2 += m * T(trk(SIDE, b, 2));
@@ -45,7 +45,7 @@ __device__ __forceinline__ void trace_eval(const T cf[8][4], T s, T p2, T p3, T 
 // LhEz.f90:6-16 (face EMF).  Face i (0..Nx) of row j (1..Ny) lies between
 // cells i and i+1.
 template <int FT, typename T>
-__global__ __launch_bounds__(BS) void k_xface(const double* __restrict__ U, T* __restrict__ FX,
+__global__ __launch_bounds__(BS) void k_xface(SRd<true> U, T* __restrict__ FX,
                                               T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
   const long long f = xcd_block() * BS + threadIdx.x;
   const long long nf = (long long)(g.Nx + 1) * g.Ny;
@@ -53,8 +53,8 @@ __global__ __launch_bounds__(BS) void k_xface(const double* __restrict__ U, T* _
   const int i = (int)(f % (g.Nx + 1)), j = 1 + (int)(f / (g.Nx + 1));
   const long long o = cix(g, i, j);
   T cL[8][4], cR[8][4];
-  trace_coef<0, T>(U, g.plane, o, cL);                 // right trace of cell i
-  trace_coef<1, T>(U, g.plane, cix(g, i + 1, j), cR);  // left trace of cell i+1
+  trace_coef<0, T>(U, o, cL);                 // right trace of cell i
+  trace_coef<1, T>(U, cix(g, i + 1, j), cR);  // left trace of cell i+1
 #pragma unroll 1
   for (int q = 0; q < 5; q++) {
     constexpr double L0 = lam(0), L1 = lam(1), L2 = lam(2), L3 = lam(3), L4 = lam(4);
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(BS) void k_xface(const double* __restrict__ U, T* _
 // y faces: Lh.f90:139-203, :237-264; LhEz.f90:18-28.  Face j (0..Ny) of
 // column i (1..Nx) lies between cells j and j+1.
 template <int FT, typename T>
-__global__ __launch_bounds__(BS) void k_yface(const double* __restrict__ U, T* __restrict__ FY,
+__global__ __launch_bounds__(BS) void k_yface(SRd<true> U, T* __restrict__ FY,
                                               T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
   const long long f = xcd_block() * BS + threadIdx.x;
   const long long nf = (long long)g.Nx * (g.Ny + 1);
@@ -89,8 +89,8 @@ __global__ __launch_bounds__(BS) void k_yface(const double* __restrict__ U, T* _
   const int i = 1 + (int)(f % g.Nx), j = (int)(f / g.Nx);
   const long long o = cix(g, i, j);
   T cD[8][4], cU[8][4];
-  trace_coef<2, T>(U, g.plane, o, cD);                 // upper trace of cell j (reference UU)
-  trace_coef<3, T>(U, g.plane, cix(g, i, j + 1), cU);  // lower trace of cell j+1 (reference UD)
+  trace_coef<2, T>(U, o, cD);                 // upper trace of cell j (reference UU)
+  trace_coef<3, T>(U, cix(g, i, j + 1), cU);  // lower trace of cell j+1 (reference UD)
 #pragma unroll 1
   for (int q = 0; q < 5; q++) {
     constexpr double L0 = lam(0), L1 = lam(1), L2 = lam(2), L3 = lam(3), L4 = lam(4);
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(BS) void k_yface(const double* __restrict__ U, T* _
 // branches of magnetically dominated states amplify that past 1e-12.
 // x faces:
 template <int FT, typename T>
-__global__ __launch_bounds__(BS) void k_xface_pt(const double* __restrict__ U, T* __restrict__ FX,
+__global__ __launch_bounds__(BS) void k_xface_pt(SRd<false> U, T* __restrict__ FX,
                                               T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
   const long long t = xcd_block() * BS + threadIdx.x;
   const long long nf = (long long)(g.Nx + 1) * g.Ny;
@@ -129,8 +129,8 @@ __global__ __launch_bounds__(BS) void k_xface_pt(const double* __restrict__ U, T
   const long long f = t / 5;
   const int i = (int)(f % (g.Nx + 1)), j = 1 + (int)(f / (g.Nx + 1));
   T UL[8], UR[8], FL[8], FR[8], F[8];
-  trace_at<0, T>(U, g.plane, cix(g, i, j), q, UL);       // right trace of cell i
-  trace_at<1, T>(U, g.plane, cix(g, i + 1, j), q, UR);   // left trace of cell i+1
+  trace_at<0, T>(U, cix(g, i, j), q, UL);       // right trace of cell i
+  trace_at<1, T>(U, cix(g, i + 1, j), q, UR);   // left trace of cell i+1
   lh_flux<T, 1>(UL, FL);
   lh_flux<T, 1>(UR, FR);
   T a1, b1, a2, b2;
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(BS) void k_xface_pt(const double* __restrict__ U, T
 
 // y faces, plain form, one lane per (face, Gauss point)
 template <int FT, typename T>
-__global__ __launch_bounds__(BS) void k_yface_pt(const double* __restrict__ U, T* __restrict__ FY,
+__global__ __launch_bounds__(BS) void k_yface_pt(SRd<false> U, T* __restrict__ FY,
                                               T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
   const long long t = xcd_block() * BS + threadIdx.x;
   const long long nf = (long long)g.Nx * (g.Ny + 1);
@@ -156,8 +156,8 @@ __global__ __launch_bounds__(BS) void k_yface_pt(const double* __restrict__ U, T
   const long long f = t / 5;
   const int i = 1 + (int)(f % g.Nx), j = (int)(f / g.Nx);
   T UD[8], UU[8], FD[8], FU[8], F[8];
-  trace_at<2, T>(U, g.plane, cix(g, i, j), q, UD);       // upper trace of cell j (reference UU)
-  trace_at<3, T>(U, g.plane, cix(g, i, j + 1), q, UU);   // lower trace of cell j+1 (reference UD)
+  trace_at<2, T>(U, cix(g, i, j), q, UD);       // upper trace of cell j (reference UU)
+  trace_at<3, T>(U, cix(g, i, j + 1), q, UU);   // lower trace of cell j+1 (reference UD)
   lh_flux<T, 2, false>(UD, FD);
   lh_flux<T, 2, true>(UU, FU);
   T a1, b1, a2, b2;
@@ -174,32 +174,32 @@ __global__ __launch_bounds__(BS) void k_yface_pt(const double* __restrict__ U, T
 
 // vertex (corner) EMF: LhEz.f90:33-69.  Vertex (i, j) is the upper-right
 // corner of cell (i, j), i = 0..Nx, j = 0..Ny.
-template <int V>
-__device__ __forceinline__ void vertex_state(const double* __restrict__ U, long long plane, long long c, double out[8]) {
+template <int V, class SV>
+__device__ __forceinline__ void vertex_state(const SV& U, long long c, double out[8]) {
 #pragma unroll
   for (int n = 0; n < 8; n++) {
     double acc = 0.0;
 #pragma unroll
     for (int b = 0; b < 10; b++) {
       const int p = PL(b, n);
-      if (p >= 0) acc += U[p * plane + c] * phiV(V, b);
+      if (p >= 0) acc += U(p, c) * phiV(V, b);
     }
     out[n] = acc;
   }
 }
 
-template <int FT>
-__global__ __launch_bounds__(BS) void k_vertex(const double* __restrict__ U, double* __restrict__ EZV, Geo g,
+template <int FT, bool MIX>
+__global__ __launch_bounds__(BS) void k_vertex(SRd<MIX> U, double* __restrict__ EZV, Geo g,
                                                const DevScal* __restrict__ sc) {
   const long long t = xcd_block() * BS + threadIdx.x;
   const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
   if (t >= nv) return;
   const int i = (int)(t % (g.Nx + 1)), j = (int)(t / (g.Nx + 1));
   double URU[8], ULU[8], URD[8], ULD[8];
-  vertex_state<3>(U, g.plane, cix(g, i + 1, j + 1), URU);  // LD corner of the upper-right cell
-  vertex_state<2>(U, g.plane, cix(g, i, j + 1), ULU);      // RD corner of the upper-left cell
-  vertex_state<1>(U, g.plane, cix(g, i + 1, j), URD);      // LU corner of the lower-right cell
-  vertex_state<0>(U, g.plane, cix(g, i, j), ULD);          // RU corner of the lower-left cell
+  vertex_state<3>(U, cix(g, i + 1, j + 1), URU);  // LD corner of the upper-right cell
+  vertex_state<2>(U, cix(g, i, j + 1), ULU);      // RD corner of the upper-left cell
+  vertex_state<1>(U, cix(g, i + 1, j), URD);      // LU corner of the lower-right cell
+  vertex_state<0>(U, cix(g, i, j), ULD);          // RU corner of the lower-left cell
   EZV[cix(g, i, j)] = vertex_emf<double, FT>(URU, ULU, URD, ULD);
 }
 
@@ -226,25 +226,34 @@ int launch_fluxes(mhd2d_ctx* ctx, const double* U, bool with_vertex, const DevSc
     {
       Timed tm(ctx, KC_XFACE);
       if (mixed)
-        hipLaunchKernelGGL((k_xface<FT, float>), dim3(nblk(nxf)), dim3(BS), 0, ctx->stream, U, (float*)ctx->FX,
+        hipLaunchKernelGGL((k_xface<FT, float>), dim3(nblk(nxf)), dim3(BS), 0, ctx->stream, SRd<true>{U, g.plane},
+                           (float*)ctx->FX,
                            (float*)ctx->EZX, g, sc);
       else
-        hipLaunchKernelGGL((k_xface_pt<FT, double>), dim3(nblk(5 * nxf)), dim3(BS), 0, ctx->stream, U, ctx->FX,
+        hipLaunchKernelGGL((k_xface_pt<FT, double>), dim3(nblk(5 * nxf)), dim3(BS), 0, ctx->stream,
+                           SRd<false>{U, g.plane}, ctx->FX,
                            ctx->EZX, g, sc);
     }
     {
       Timed tm(ctx, KC_YFACE);
       if (mixed)
-        hipLaunchKernelGGL((k_yface<FT, float>), dim3(nblk(nyf)), dim3(BS), 0, ctx->stream, U, (float*)ctx->FY,
+        hipLaunchKernelGGL((k_yface<FT, float>), dim3(nblk(nyf)), dim3(BS), 0, ctx->stream, SRd<true>{U, g.plane},
+                           (float*)ctx->FY,
                            (float*)ctx->EZY, g, sc);
       else
-        hipLaunchKernelGGL((k_yface_pt<FT, double>), dim3(nblk(5 * nyf)), dim3(BS), 0, ctx->stream, U, ctx->FY,
+        hipLaunchKernelGGL((k_yface_pt<FT, double>), dim3(nblk(5 * nyf)), dim3(BS), 0, ctx->stream,
+                           SRd<false>{U, g.plane}, ctx->FY,
                            ctx->EZY, g, sc);
     }
     if (with_vertex) {
       Timed tm(ctx, KC_VERTEX);
       const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
-      hipLaunchKernelGGL(k_vertex<FT>, dim3(nblk(nv)), dim3(BS), 0, ctx->stream, U, ctx->EZV, g, sc);
+      if (mixed)
+        hipLaunchKernelGGL((k_vertex<FT, true>), dim3(nblk(nv)), dim3(BS), 0, ctx->stream, SRd<true>{U, g.plane},
+                           ctx->EZV, g, sc);
+      else
+        hipLaunchKernelGGL((k_vertex<FT, false>), dim3(nblk(nv)), dim3(BS), 0, ctx->stream, SRd<false>{U, g.plane},
+                           ctx->EZV, g, sc);
     }
     LAUNCHCHK(ctx);
     return 0;
@@ -258,7 +267,12 @@ int launch_vertex(mhd2d_ctx* ctx, const double* U, const DevScal* sc) {
     constexpr int FT = decltype(FTc)::value;
     const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
     Timed tm(ctx, KC_VERTEX);
-    hipLaunchKernelGGL(k_vertex<FT>, dim3(nblk(nv)), dim3(BS), 0, ctx->stream, U, ctx->EZV, g, sc);
+    if (ctx->cfg.precision == MHD2D_MIXED)
+      hipLaunchKernelGGL((k_vertex<FT, true>), dim3(nblk(nv)), dim3(BS), 0, ctx->stream, SRd<true>{U, g.plane},
+                         ctx->EZV, g, sc);
+    else
+      hipLaunchKernelGGL((k_vertex<FT, false>), dim3(nblk(nv)), dim3(BS), 0, ctx->stream, SRd<false>{U, g.plane},
+                         ctx->EZV, g, sc);
     LAUNCHCHK(ctx);
     return 0;
   });

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -65,9 +66,8 @@ __device__ __forceinline__ double sel5(int q, double a0, double a1, double a2, d
 }
 
 // trace of all 8 variables of one cell on one side at node q, in type T
-template <int SIDE, typename T>
-__device__ __forceinline__ void trace_at(const double* __restrict__ U, long long plane, long long c, int q,
-                                         T out[8]) {
+template <int SIDE, typename T, class SV>
+__device__ __forceinline__ void trace_at(const SV& U, long long c, int q, T out[8]) {
   constexpr double L0 = lam(0), L1 = lam(1), L2 = lam(2), L3 = lam(3), L4 = lam(4);
   const double s = sel5(q, L0, L1, L2, L3, L4);
   const double p2 = sel5(q, P2(L0), P2(L1), P2(L2), P2(L3), P2(L4));
@@ -79,7 +79,7 @@ __device__ __forceinline__ void trace_at(const double* __restrict__ U, long long
 #pragma unroll
     for (int b = 0; b < 10; b++) {
       const int p = PL(b, n);
-      if (p >= 0) acc += T(U[p * plane + c]) * T(phiTr(SIDE, b, s, p2, p3, t23));
+      if (p >= 0) acc += T(U(p, c)) * T(phiTr(SIDE, b, s, p2, p3, t23));
     }
     out[n] = acc;
   }
@@ -117,9 +117,11 @@ struct mhd2d_ctx {
   mhd::Geo g;
   hipStream_t stream;
   int device;
-  // state buffers (52 planes each); roles rotate inside mhd2d_step
+  // state buffers (52 planes each, or the mixed-form storage of
+  // mhd2d_tables.hpp); roles rotate inside mhd2d_step
   double *S, *U0, *UI, *UII, *T;
   double* du;
+  double* stage;       // mixed form only: 52 FP64 planes for upload/download (allocated on first use)
   // face buffers (6 planes each)
   double *F, *F0, *FI, *FII, *Ft, *dEz;
   // face fluxes and EMFs
@@ -202,6 +204,12 @@ struct Timed {
   }
 };
 
+// calls f(std::true_type) for the mixed-form state storage, f(std::false_type) otherwise
+template <typename F>
+int with_mix(const mhd2d_ctx* ctx, F&& f) {
+  return ctx->cfg.precision == MHD2D_MIXED ? f(std::true_type{}) : f(std::false_type{});
+}
+
 // launch functions (stream-ordered on ctx->stream; 0 or a negative status)
 int launch_fluxes(mhd2d_ctx* ctx, const double* U, bool with_vertex, const DevScal* sc);     // mhd2d_faces.hip
 int launch_vertex(mhd2d_ctx* ctx, const double* U, const DevScal* sc);                      // mhd2d_faces.hip
@@ -211,6 +219,8 @@ int launch_ct(mhd2d_ctx* ctx, int mode, const double* F, const double* A, const 
               const DevScal* sc, double dt);                                                  // mhd2d_cell.hip
 int launch_axpy(mhd2d_ctx* ctx, double* out, const double* X, double a, const double* Y, double b,
                 const double* Z, double dt, long long n);                                     // mhd2d_cell.hip
+int launch_axpy_state(mhd2d_ctx* ctx, double* out, const double* X, double a, const double* Y, double b,
+                      const double* Z, double dt);                                            // mhd2d_cell.hip
 int launch_kxrcf(mhd2d_ctx* ctx, const double* U, DevScal* sc);                               // mhd2d_limit.hip
 int launch_tvb_list(mhd2d_ctx* ctx, const double* in);                                       // mhd2d_limit.hip
 int launch_detect_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc = nullptr);              // mhd2d_limit.hip

@@ -160,8 +160,8 @@ __device__ __forceinline__ bool is_bplane(int p) { return (p >= 30 && p < 42) ||
 // the whole array: interior cells are limited, halo cells copied (or, with
 // DIV, their B planes zeroed as div_free_Balsara.f90:13 does).
 // flags: indicator == 2/3 reads trouble[] (1 = limit).
-template <bool APPLY, bool PP, bool DIV>
-__global__ __launch_bounds__(BS) void k_limit(const double* __restrict__ in, double* __restrict__ out,
+template <bool APPLY, bool PP, bool DIV, bool MIX>
+__global__ __launch_bounds__(BS) void k_limit(SRd<MIX> in, SWr<MIX> out,
                                               const double* __restrict__ F, const int* __restrict__ tmask,
                                               const double* __restrict__ trec, Geo g) {
   const long long t = xcd_block() * BS + threadIdx.x;
@@ -172,12 +172,12 @@ __global__ __launch_bounds__(BS) void k_limit(const double* __restrict__ in, dou
   const bool interior = i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny;
   if (!interior) {
 #pragma unroll 4
-    for (int p = 0; p < NPL; p++) out[p * pl + c] = (DIV && is_bplane(p)) ? 0.0 : in[p * pl + c];
+    for (int p = 0; p < NPL; p++) out.set(p, c, (DIV && is_bplane(p)) ? 0.0 : in(p, c));
     return;
   }
   double m[NPL];
 #pragma unroll
-  for (int p = 0; p < NPL; p++) m[p] = in[p * pl + c];
+  for (int p = 0; p < NPL; p++) m[p] = in(p, c);
   if (APPLY) {
     // TVB_Limiter_P2.f90:106-114 for the variables k_tvb marked changed
     const long long ncell = (long long)g.Nx * g.Ny, tc = (long long)(j - 1) * g.Nx + (i - 1);
@@ -204,11 +204,12 @@ __global__ __launch_bounds__(BS) void k_limit(const double* __restrict__ in, dou
   if (PP) pp_cell(m);
   if (DIV) divfree_cell(m, F, g, i, j);
 #pragma unroll
-  for (int p = 0; p < NPL; p++) out[p * pl + c] = m[p];
+  for (int p = 0; p < NPL; p++) out.set(p, c, m[p]);
 }
 
 
-__global__ __launch_bounds__(BS) void k_kxrcf(const double* __restrict__ U, int* __restrict__ trouble, Geo g,
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_kxrcf(SRd<MIX> U, int* __restrict__ trouble, Geo g,
                                               double Ck, double hk, DevScal* sc) {
   const long long t = (long long)blockIdx.x * BS + threadIdx.x;
   if (t >= (long long)g.Nx * g.Ny) return;
@@ -224,7 +225,8 @@ __global__ void k_list_reset(DevScal* lst) {
 
 // KXRCF over the interior cells for the limiter pass: trouble[] and the
 // compacted list of flagged cells (one atomic per wave), tmask[] cleared
-__global__ __launch_bounds__(BS) void k_detect(const double* __restrict__ U, int* __restrict__ trouble,
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_detect(SRd<MIX> U, int* __restrict__ trouble,
                                                int* __restrict__ tmask, int* __restrict__ list, Geo g, double Ck,
                                                double hk, DevScal* sc, DevScal* lst) {
   const long long t = xcd_block() * BS + threadIdx.x;
@@ -240,21 +242,22 @@ __global__ __launch_bounds__(BS) void k_detect(const double* __restrict__ U, int
 // TVB decision and edge values of one cell (TVB_Limiter_P2.f90:24-104):
 // returns the change bits per variable; Rm/Lm (x) and Um/Dm (y) are the
 // modified edge deviations DeltaUR1mod .. DeltaUD1mod.
-__device__ __forceinline__ int tvb_cell(const double* __restrict__ in, const Geo& g, int i, int j, double M,
+template <class SV>
+__device__ __forceinline__ int tvb_cell(const SV& in, const Geo& g, int i, int j, double M,
                                         double Rm[8], double Lm[8], double Um[8], double Dm[8]) {
   const long long pl = g.plane, c = cix(g, i, j);
   int change = 0;
   double U[8];
 #pragma unroll
-  for (int n = 0; n < 8; n++) U[n] = in[PL(0, n) * pl + c];
+  for (int n = 0; n < 8; n++) U[n] = in(PL(0, n), c);
   {
     double dR[8], dL[8], r1[8], l1[8];
     const long long cE = cix(g, i + 1, j), cW = cix(g, i - 1, j);
 #pragma unroll
     for (int n = 0; n < 8; n++) {
-      const double m1 = in[PL(1, n) * pl + c], m3 = in[PL(3, n) * pl + c];
-      dR[n] = in[PL(0, n) * pl + cE] - U[n];
-      dL[n] = U[n] - in[PL(0, n) * pl + cW];
+      const double m1 = in(PL(1, n), c), m3 = in(PL(3, n), c);
+      dR[n] = in(PL(0, n), cE) - U[n];
+      dL[n] = U[n] - in(PL(0, n), cW);
       r1[n] = m1 + (2.0 / 3.0) * m3;
       l1[n] = m1 - (2.0 / 3.0) * m3;
     }
@@ -268,9 +271,9 @@ __device__ __forceinline__ int tvb_cell(const double* __restrict__ in, const Geo
     const long long cN = cix(g, i, j + 1), cS = cix(g, i, j - 1);
 #pragma unroll
     for (int n = 0; n < 8; n++) {
-      const double m2 = in[PL(2, n) * pl + c], m5 = in[PL(5, n) * pl + c];
-      dR[n] = in[PL(0, n) * pl + cN] - U[n];
-      dL[n] = U[n] - in[PL(0, n) * pl + cS];
+      const double m2 = in(PL(2, n), c), m5 = in(PL(5, n), c);
+      dR[n] = in(PL(0, n), cN) - U[n];
+      dL[n] = U[n] - in(PL(0, n), cS);
       r1[n] = m2 + (2.0 / 3.0) * m5;
       l1[n] = m2 - (2.0 / 3.0) * m5;
     }
@@ -290,8 +293,8 @@ __device__ __forceinline__ int tvb_cell(const double* __restrict__ in, const Geo
 // apart from the detector and from k_limit so that the eigen-decompositions'
 // register demand does not lower the occupancy of the passes every cell
 // takes.
-template <int IND>
-__global__ __launch_bounds__(BS) void k_tvb(const double* __restrict__ in, int* __restrict__ trouble,
+template <int IND, bool MIX>
+__global__ __launch_bounds__(BS) void k_tvb(SRd<MIX> in, int* __restrict__ trouble,
                                             int* __restrict__ tmask, double* __restrict__ trec, Geo g, double M,
                                             double Ck, double hk, DevScal* sc) {
   const long long ncell = (long long)g.Nx * g.Ny;
@@ -318,7 +321,8 @@ __global__ __launch_bounds__(BS) void k_tvb(const double* __restrict__ in, int* 
 }
 
 // TVB on the listed (flagged) cells only; a fixed grid strides the list
-__global__ __launch_bounds__(BS) void k_tvb_list(const double* __restrict__ in, const int* __restrict__ list,
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_tvb_list(SRd<MIX> in, const int* __restrict__ list,
                                                  int* __restrict__ tmask, double* __restrict__ trec, Geo g,
                                                  double M, const DevScal* __restrict__ lst) {
   const long long ncell = (long long)g.Nx * g.Ny;
@@ -346,20 +350,26 @@ __global__ __launch_bounds__(BS) void k_tvb_list(const double* __restrict__ in, 
 int launch_kxrcf(mhd2d_ctx* ctx, const double* U, DevScal* sc) {
   const Geo& g = ctx->g;
   Timed tm(ctx, KC_KXRCF);
-  hipLaunchKernelGGL(k_kxrcf, dim3(nblk((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream, U, ctx->trouble, g,
-                     ctx->cfg.Ck, kx_hk(g), sc);
-  LAUNCHCHK(ctx);
-  return 0;
+  return with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_kxrcf<MIX>), dim3(nblk((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream,
+                       SRd<MIX>{U, g.plane}, ctx->trouble, g, ctx->cfg.Ck, kx_hk(g), sc);
+    LAUNCHCHK(ctx);
+    return 0;
+  });
 }
 
 // TVB over the compacted list of flagged cells (a fixed grid strides it)
 int launch_tvb_list(mhd2d_ctx* ctx, const double* in) {
   const Geo& g = ctx->g;
   const unsigned nb = std::min<unsigned>(nblk((long long)g.Nx * g.Ny), 1024u);
-  hipLaunchKernelGGL(k_tvb_list, dim3(nb), dim3(BS), 0, ctx->stream, in, (const int*)ctx->tlist, ctx->tmask,
-                     ctx->trec, g, ctx->cfg.M, (const DevScal*)ctx->sc);
-  LAUNCHCHK(ctx);
-  return 0;
+  return with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_tvb_list<MIX>), dim3(nb), dim3(BS), 0, ctx->stream, SRd<MIX>{in, g.plane},
+                       (const int*)ctx->tlist, ctx->tmask, ctx->trec, g, ctx->cfg.M, (const DevScal*)ctx->sc);
+    LAUNCHCHK(ctx);
+    return 0;
+  });
 }
 
 // KXRCF with a compacted list, then TVB over the list: trouble[] = KXRCF
@@ -367,9 +377,15 @@ int launch_tvb_list(mhd2d_ctx* ctx, const double* in) {
 int launch_detect_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc) {
   const Geo& g = ctx->g;
   hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc);
-  hipLaunchKernelGGL(k_detect, dim3(nblk((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream, in, ctx->trouble,
-                     ctx->tmask, ctx->tlist, g, ctx->cfg.Ck, kx_hk(g), sc, ctx->sc);
-  LAUNCHCHK(ctx);
+  int rc = with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_detect<MIX>), dim3(nblk((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream,
+                       SRd<MIX>{in, g.plane}, ctx->trouble, ctx->tmask, ctx->tlist, g, ctx->cfg.Ck, kx_hk(g), sc,
+                       ctx->sc);
+    LAUNCHCHK(ctx);
+    return 0;
+  });
+  if (rc) return rc;
   return launch_tvb_list(ctx, in);
 }
 
@@ -378,11 +394,15 @@ int launch_detect_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc) {
 int launch_tvb_all(mhd2d_ctx* ctx, const double* in, int* kx) {
   const Geo& g = ctx->g;
   const dim3 gt(nblk((long long)g.Nx * g.Ny)), bt(BS);
-  hipLaunchKernelGGL(k_kxrcf, gt, bt, 0, ctx->stream, in, kx, g, ctx->cfg.Ck, kx_hk(g), (DevScal*)nullptr);
-  hipLaunchKernelGGL(k_tvb<MHD2D_IND_ALL>, gt, bt, 0, ctx->stream, in, ctx->trouble, ctx->tmask, ctx->trec, g,
-                     ctx->cfg.M, ctx->cfg.Ck, kx_hk(g), (DevScal*)nullptr);
-  LAUNCHCHK(ctx);
-  return 0;
+  return with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_kxrcf<MIX>), gt, bt, 0, ctx->stream, SRd<MIX>{in, g.plane}, kx, g, ctx->cfg.Ck, kx_hk(g),
+                       (DevScal*)nullptr);
+    hipLaunchKernelGGL((k_tvb<MHD2D_IND_ALL, MIX>), gt, bt, 0, ctx->stream, SRd<MIX>{in, g.plane}, ctx->trouble,
+                       ctx->tmask, ctx->trec, g, ctx->cfg.M, ctx->cfg.Ck, kx_hk(g), (DevScal*)nullptr);
+    LAUNCHCHK(ctx);
+    return 0;
+  });
 }
 
 // limiter pass in -> out: detector + TVB decisions (k_tvb), then one pass
@@ -398,9 +418,15 @@ int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F,
   if (tvb) {
     int rc = 0;
     if (ind == MHD2D_IND_ALL) {
-      hipLaunchKernelGGL(k_tvb<MHD2D_IND_ALL>, dim3(nblk((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream, in,
-                         ctx->trouble, ctx->tmask, ctx->trec, g, ctx->cfg.M, ctx->cfg.Ck, kx_hk(g), sc);
-      LAUNCHCHK(ctx);
+      rc = with_mix(ctx, [&](auto Mc) -> int {
+        constexpr bool MIX = decltype(Mc)::value;
+        hipLaunchKernelGGL((k_tvb<MHD2D_IND_ALL, MIX>), dim3(nblk((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream,
+                           SRd<MIX>{in, g.plane}, ctx->trouble, ctx->tmask, ctx->trec, g, ctx->cfg.M, ctx->cfg.Ck,
+                           kx_hk(g), sc);
+        LAUNCHCHK(ctx);
+        return 0;
+      });
+      if (rc) return rc;
     } else if (ind == MHD2D_IND_MLP) {
       // learned screen (once per step), then KXRCF + TVB on its candidates
       if (stage <= 1 && (rc = launch_mlp_screen(ctx, in))) return rc;
@@ -410,19 +436,24 @@ int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F,
     }
   }
   const dim3 gr(nblk(n)), bl(BS);
+  return with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    const SRd<MIX> vi{in, g.plane};
+    const SWr<MIX> vo{out, g.plane};
 #define LIM(a, b, c) \
-  hipLaunchKernelGGL((k_limit<a, b, c>), gr, bl, 0, ctx->stream, in, out, F, ctx->tmask, ctx->trec, g)
-  if (tvb && pp && div) LIM(true, true, true);
-  else if (tvb && pp) LIM(true, true, false);
-  else if (tvb && div) LIM(true, false, true);
-  else if (tvb) LIM(true, false, false);
-  else if (pp && div) LIM(false, true, true);
-  else if (pp) LIM(false, true, false);
-  else if (div) LIM(false, false, true);
-  else LIM(false, false, false);
+    hipLaunchKernelGGL((k_limit<a, b, c, MIX>), gr, bl, 0, ctx->stream, vi, vo, F, ctx->tmask, ctx->trec, g)
+    if (tvb && pp && div) LIM(true, true, true);
+    else if (tvb && pp) LIM(true, true, false);
+    else if (tvb && div) LIM(true, false, true);
+    else if (tvb) LIM(true, false, false);
+    else if (pp && div) LIM(false, true, true);
+    else if (pp) LIM(false, true, false);
+    else if (div) LIM(false, false, true);
+    else LIM(false, false, false);
 #undef LIM
-  LAUNCHCHK(ctx);
-  return 0;
+    LAUNCHCHK(ctx);
+    return 0;
+  });
 }
 
 }  // namespace mhd2d_impl

@@ -42,7 +42,8 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int XP = 68;          // LDS row pitch (bf16) of a 64-wide activation row: conflict-free A reads
 
-__device__ __forceinline__ void mlp_features(const double* __restrict__ U, const Geo& g, int i, int j,
+template <class SV>
+__device__ __forceinline__ void mlp_features(const SV& U, const Geo& g, int i, int j,
                                              double Mh2, float f[MLP_NIN]) {
   const long long pl = g.plane;
   const long long cs[5] = {cix(g, i, j), cix(g, i + 1, j), cix(g, i - 1, j), cix(g, i, j + 1), cix(g, i, j - 1)};
@@ -51,10 +52,10 @@ __device__ __forceinline__ void mlp_features(const double* __restrict__ U, const
   double sr = 0.0, se = 0.0, sm = 0.0;
 #pragma unroll
   for (int k = 0; k < 5; k++) {
-    r[k] = U[PL(0, 0) * pl + cs[k]];
-    e[k] = U[PL(0, 4) * pl + cs[k]];
-    mu[k] = U[PL(0, 1) * pl + cs[k]];
-    mv[k] = U[PL(0, 2) * pl + cs[k]];
+    r[k] = U(PL(0, 0), cs[k]);
+    e[k] = U(PL(0, 4), cs[k]);
+    mu[k] = U(PL(0, 1), cs[k]);
+    mv[k] = U(PL(0, 2), cs[k]);
     sr = fmax(sr, fabs(r[k]));
     se = fmax(se, fabs(e[k]));
     sm = fmax(sm, fmax(fabs(mu[k]), fabs(mv[k])));
@@ -67,8 +68,8 @@ __device__ __forceinline__ void mlp_features(const double* __restrict__ U, const
     o[5] = k == 0 ? (float)(log10(sqrt(g.hx * g.hy)) * 0.5) : (float)((e[k] - e[0]) * ie);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      o[1 + q] = (float)(U[PL(dm[q], 0) * pl + cs[k]] * ir);
-      o[6 + q] = (float)(U[PL(dm[q], 4) * pl + cs[k]] * ie);
+      o[1 + q] = (float)(U(PL(dm[q], 0), cs[k]) * ir);
+      o[6 + q] = (float)(U(PL(dm[q], 4), cs[k]) * ie);
     }
     o[10] = (float)(mu[k] * im);
     o[11] = (float)(mv[k] * im);
@@ -89,7 +90,8 @@ __device__ __forceinline__ void band_append(bool band, int t, int* __restrict__ 
 
 // the same features in FP32 for inference (l10Mh2 = log10(M hx hy),
 // l10h = log10(sqrt(hx hy)) / 2 precomputed)
-__device__ __forceinline__ void mlp_features_f32(const double* __restrict__ U, const Geo& g, int i, int j,
+template <class SV>
+__device__ __forceinline__ void mlp_features_f32(const SV& U, const Geo& g, int i, int j,
                                                  float l10Mh2, float l10h, float f[MLP_NIN]) {
   const long long pl = g.plane;
   const long long cs[5] = {cix(g, i, j), cix(g, i + 1, j), cix(g, i - 1, j), cix(g, i, j + 1), cix(g, i, j - 1)};
@@ -98,10 +100,10 @@ __device__ __forceinline__ void mlp_features_f32(const double* __restrict__ U, c
   float sr = 0.f, se = 0.f, sm = 0.f;
 #pragma unroll
   for (int k = 0; k < 5; k++) {
-    r[k] = (float)U[PL(0, 0) * pl + cs[k]];
-    e[k] = (float)U[PL(0, 4) * pl + cs[k]];
-    mu[k] = (float)U[PL(0, 1) * pl + cs[k]];
-    mv[k] = (float)U[PL(0, 2) * pl + cs[k]];
+    r[k] = (float)U(PL(0, 0), cs[k]);
+    e[k] = (float)U(PL(0, 4), cs[k]);
+    mu[k] = (float)U(PL(0, 1), cs[k]);
+    mv[k] = (float)U(PL(0, 2), cs[k]);
     sr = fmaxf(sr, fabsf(r[k]));
     se = fmaxf(se, fabsf(e[k]));
     sm = fmaxf(sm, fmaxf(fabsf(mu[k]), fabsf(mv[k])));
@@ -115,8 +117,8 @@ __device__ __forceinline__ void mlp_features_f32(const double* __restrict__ U, c
     o[5] = k == 0 ? l10h : (e[k] - e[0]) * ie;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      o[1 + q] = (float)U[PL(dm[q], 0) * pl + cs[k]] * ir;
-      o[6 + q] = (float)U[PL(dm[q], 4) * pl + cs[k]] * ie;
+      o[1 + q] = (float)U(PL(dm[q], 0), cs[k]) * ir;
+      o[6 + q] = (float)U(PL(dm[q], 4), cs[k]) * ie;
     }
     o[10] = mu[k] * im;
     o[11] = mv[k] * im;
@@ -157,7 +159,8 @@ struct MlpDev {
   float b1[64], b2[64], w3[64], b3;
 };
 
-__global__ __launch_bounds__(256) void k_mlp_screen(const double* __restrict__ U, int* __restrict__ screen,
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_mlp_screen(SRd<MIX> U, int* __restrict__ screen,
                                                     float* __restrict__ prob, const MlpDev* __restrict__ W,
                                                     Geo g, float l10Mh2, float l10h, float lo) {
   __shared__ __attribute__((aligned(16))) unsigned short sX[4][64 * XP];
@@ -277,7 +280,8 @@ __global__ void k_reset_nlist(DevScal* lst) {
 }
 
 // KXRCF_Detector on the candidates; flagged ones join the troubled-cell list
-__global__ __launch_bounds__(BS) void k_kxrcf_cands(const double* __restrict__ U, int* __restrict__ trouble,
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_kxrcf_cands(SRd<MIX> U, int* __restrict__ trouble,
                                                     int* __restrict__ tmask, int* __restrict__ list,
                                                     const int* __restrict__ blist, Geo g, double Ck, double hk,
                                                     DevScal* sc, DevScal* lst) {
@@ -311,7 +315,8 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
   return z ^ (z >> 31);
 }
 
-__global__ __launch_bounds__(BS) void k_collect(const double* __restrict__ U, const int* __restrict__ trouble,
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_collect(SRd<MIX> U, const int* __restrict__ trouble,
                                                 const int* __restrict__ tmask, Geo g, double Mh2, double neg_keep,
                                                 unsigned long long seed, float* __restrict__ X,
                                                 float* __restrict__ Y, long long* __restrict__ idx,
@@ -496,11 +501,15 @@ static int launch_screen_only(mhd2d_ctx* ctx, const double* in) {
   const Geo& g = ctx->g;
   if (!ctx->mlpw) return fail(ctx, -6, "indicator MLP: no weights (mhd2d_mlp_set_weights)");
   const long long ncell = (long long)g.Nx * g.Ny;
-  hipLaunchKernelGGL(k_mlp_screen, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, ctx->stream, in,
-                     ctx->screen, ctx->prob, (const MlpDev*)ctx->mlpw, g, (float)log10(ctx->cfg.M * g.hx * g.hy),
-                     (float)(0.5 * log10(sqrt(g.hx * g.hy))), (float)ctx->cfg.mlp_lo);
-  LAUNCHCHK(ctx);
-  return 0;
+  return with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_mlp_screen<MIX>), dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, ctx->stream,
+                       SRd<MIX>{in, g.plane}, ctx->screen, ctx->prob, (const MlpDev*)ctx->mlpw, g,
+                       (float)log10(ctx->cfg.M * g.hx * g.hy), (float)(0.5 * log10(sqrt(g.hx * g.hy))),
+                       (float)ctx->cfg.mlp_lo);
+    LAUNCHCHK(ctx);
+    return 0;
+  });
 }
 
 int launch_mlp_screen(mhd2d_ctx* ctx, const double* in) {
@@ -519,9 +528,14 @@ int launch_candidates_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc) {
   const long long ncell = (long long)g.Nx * g.Ny;
   const unsigned nb = std::min<unsigned>(nb_blocks(ncell), 1024u);
   hipLaunchKernelGGL(k_reset_nlist, dim3(1), dim3(64), 0, ctx->stream, ctx->sc);
-  hipLaunchKernelGGL(k_kxrcf_cands, dim3(nb), dim3(BS), 0, ctx->stream, in, ctx->trouble, ctx->tmask, ctx->tlist,
-                     (const int*)ctx->blist, g, ctx->cfg.Ck, kx_hk(g), sc, ctx->sc);
-  LAUNCHCHK(ctx);
+  int rc = with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_kxrcf_cands<MIX>), dim3(nb), dim3(BS), 0, ctx->stream, SRd<MIX>{in, g.plane}, ctx->trouble,
+                       ctx->tmask, ctx->tlist, (const int*)ctx->blist, g, ctx->cfg.Ck, kx_hk(g), sc, ctx->sc);
+    LAUNCHCHK(ctx);
+    return 0;
+  });
+  if (rc) return rc;
   return launch_tvb_list(ctx, in);
 }
 
@@ -699,9 +713,14 @@ int mhd2d_trainer_collect(mhd2d_trainer* tr, mhd2d_ctx* ctx, double neg_keep, ui
   // the plain method's decision on the current state: KXRCF flags + TVB change bits
   int rc = launch_detect_tvb(ctx, ctx->S);
   if (rc) return tfail(tr, rc, "collect: %s", ctx->err);
-  hipLaunchKernelGGL(k_collect, dim3(nb_of((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream, (const double*)ctx->S,
-                     (const int*)ctx->trouble, (const int*)ctx->tmask, g, ctx->cfg.M * g.hx * g.hy, neg_keep,
-                     (unsigned long long)seed, tr->X, tr->Y, tr->idx, tr->cnt, tr->cnt + 1, tr->cap);
+  rc = with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_collect<MIX>), dim3(nb_of((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream,
+                       SRd<MIX>{ctx->S, g.plane}, (const int*)ctx->trouble, (const int*)ctx->tmask, g,
+                       ctx->cfg.M * g.hx * g.hy, neg_keep, (unsigned long long)seed, tr->X, tr->Y, tr->idx, tr->cnt,
+                       tr->cnt + 1, tr->cap);
+    return 0;
+  });
   unsigned long long h[2];
   TCHK(tr, hipMemcpyAsync(h, tr->cnt, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   TCHK(tr, hipStreamSynchronize(ctx->stream));

@@ -5,6 +5,7 @@
 // (same formulas, round-to-nearest per operation, no contraction), so that
 // fully unrolled kernels fold them to the same doubles the reference holds.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -150,6 +151,19 @@ __host__ __device__ constexpr int PL(int b, int n) {
                             : -1;
 }
 
+// Storage of a state buffer.  Plain form: the 52 planes in FP64.  Mixed
+// form: the 22 planes of the cell averages and of both magnetic-field
+// components in FP64 (slots 0..21), then the 30 higher-mode planes of the
+// other six variables in FP32 (296 instead of 416 bytes per cell).  The
+// cell averages and every B plane stay FP64 because the averages carry the
+// conservation and the B planes the discrete divergence.
+constexpr int NPL_D = 22, NPL_F = 30;
+__host__ __device__ constexpr bool pl_f64(int p) { return (p < 48 && p % 6 == 0) || (p >= 30 && p < 42) || p >= 48; }
+__host__ __device__ constexpr int slot_d(int p) {
+  return p < 30 ? p / 6 : p < 42 ? 5 + (p - 30) : p == 42 ? 17 : 18 + (p - 48);
+}
+__host__ __device__ constexpr int slot_f(int p) { return p < 30 ? (p / 6) * 5 + (p % 6 - 1) : 25 + (p - 43); }
+
 struct Geo {
   int Nx, Ny, P;
   int bcR, bcL, bcU, bcD;
@@ -159,6 +173,45 @@ struct Geo {
 };
 __host__ __device__ __forceinline__ long long cix(const Geo& g, int i, int j) {
   return (long long)j * g.P + OFF + i;
+}
+
+// view of a state buffer (MIX: the mixed-form storage above); p is the plane
+// of PL(), c the cell index of cix()
+template <bool MIX>
+struct SRd {
+  const double* b;
+  long long pl;
+  __device__ __forceinline__ double operator()(int p, long long c) const {
+    if (!MIX) return b[(long long)p * pl + c];
+    if (pl_f64(p)) return b[(long long)slot_d(p) * pl + c];
+    return (double)reinterpret_cast<const float*>(b + NPL_D * pl)[(long long)slot_f(p) * pl + c];
+  }
+};
+template <bool MIX>
+struct SWr {
+  double* b;
+  long long pl;
+  __device__ __forceinline__ double operator()(int p, long long c) const { return SRd<MIX>{b, pl}(p, c); }
+  __device__ __forceinline__ void set(int p, long long c, double v) const {
+    if (!MIX) b[(long long)p * pl + c] = v;
+    else if (pl_f64(p)) b[(long long)slot_d(p) * pl + c] = v;
+    else reinterpret_cast<float*>(b + NPL_D * pl)[(long long)slot_f(p) * pl + c] = (float)v;
+  }
+  __device__ __forceinline__ operator SRd<MIX>() const { return SRd<MIX>{b, pl}; }
+};
+// f(integral_constant<int, I>) for I = B..E-1: a loop whose index is a
+// compile-time constant in every iteration (so the storage-map branches of
+// SRd/SWr fold away)
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+// bytes of one state buffer
+__host__ __device__ constexpr long long state_bytes(bool mix, long long plane) {
+  return mix ? (NPL_D * 8LL + NPL_F * 4LL) * plane : 52LL * 8 * plane;
 }
 
 }  // namespace mhd

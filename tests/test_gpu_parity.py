@@ -302,8 +302,8 @@ def test_invariants_2048(pkg):
 
 @pytest.mark.parametrize("N,steps", [(256, 20)])
 def test_mixed_vs_fp64(pkg, N, steps):
-    """mixed form (FP32 face fluxes and quadratures, FP64 averages, CT,
-    div-free, limiters) against the plain FP64 form on the same input:
+    """mixed form (FP32 face fluxes, quadratures and higher-mode storage;
+    FP64 averages, magnetic modes, CT, div-free, limiters) against the plain FP64 form on the same input:
     relative L2 of every field <= 1e-5 (density is the headline), div B at
     roundoff"""
     dom = (0.0, 2 * np.pi, 0.0, 2 * np.pi)
@@ -323,3 +323,29 @@ def test_mixed_vs_fp64(pkg, N, steps):
         if np.linalg.norm(b) > 0:
             assert rel_l2(a, b) < 1e-5, n
     assert d32["totaldiv"] < 1e-12
+
+
+def test_mixed_storage_roundtrip(pkg):
+    """mixed-form state storage: cell averages and every magnetic-field mode
+    stay FP64 (exact round trip), the higher modes of rho, rho u, rho v,
+    rho w, E and Bz are stored in FP32 (round to nearest)"""
+    N = 24
+    rng = np.random.default_rng(5)
+    s64 = pkg.Solver(N, N, flux_type=pkg.HLLD)
+    uh = rng.standard_normal((N + 2, N + 2, 10, 8))
+    _, sx, sy = s64.shapes()
+    Bx, By = rng.standard_normal(sx), rng.standard_normal(sy)
+    s64.upload(uh, Bx, By)
+    ref = s64.download()[0]                 # inactive modes read back as zero
+    assert np.count_nonzero(np.any(ref != 0, axis=(0, 1))) == 52
+    sm = pkg.Solver(N, N, flux_type=pkg.HLLD, precision=pkg.MIXED)
+    sm.upload(ref, Bx, By)
+    got, gx, gy = sm.download()
+    assert np.array_equal(gx, Bx) and np.array_equal(gy, By)
+    for n in range(8):
+        for b in range(10):
+            a, r = got[:, :, b, n], ref[:, :, b, n]
+            if b == 0 or n in (5, 6):
+                assert np.array_equal(a, r), (b, n)
+            else:
+                assert np.array_equal(a, r.astype(np.float32).astype(np.float64)), (b, n)
