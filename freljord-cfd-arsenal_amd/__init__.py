@@ -24,6 +24,8 @@ LF, HLL, HLLC, HLLD = 1, 2, 3, 4
 IND_NONE, IND_ALL, IND_KXRCF, IND_MLP = 0, 1, 2, 3
 FP64, MIXED = 0, 1
 IC_VORTEX, IC_OT, IC_ROTOR, IC_OT_PERTURBED, IC_RIEMANN4 = 1, 3, 4, 103, 104
+IC_BLAST, IC_CLOUD_SHOCK, IC_CLOUD_SHOCK_2, IC_MAGNETIC_LOOP = 5, 6, 61, 7
+IC_BRIO_WU, IC_BRIO_WU_Y, IC_RYU_JONES, IC_RYU_JONES_Y = 8, 81, 9, 91
 DU, DEZ1, DEZ2, EZVERTEX, TROUBLE = 0, 1, 2, 3, 4
 
 # exported symbols of include/mhd2d.h
@@ -38,6 +40,8 @@ EXPORTS = (
     "mhd2d_trainer_create", "mhd2d_trainer_destroy", "mhd2d_trainer_last_error", "mhd2d_trainer_collect",
     "mhd2d_trainer_size", "mhd2d_trainer_clear", "mhd2d_trainer_get_data", "mhd2d_trainer_set_data",
     "mhd2d_trainer_grad", "mhd2d_trainer_fit",
+    "mhd2d_checkpoint_save", "mhd2d_checkpoint_load", "mhd2d_ic_defaults", "mhd2d_calculate_az",
+    "mhd2d_calculate_l2error",
 )
 MLP_NPARAM = 8129        # W1[60][64], b1[64], W2[64][64], b2[64], w3[64], b3
 MLP_NIN = 60
@@ -55,6 +59,16 @@ class Config(C.Structure):
         ("xa", C.c_double), ("xb", C.c_double), ("ya", C.c_double), ("yb", C.c_double),
         ("M", C.c_double), ("CFL", C.c_double), ("Ck", C.c_double),
         ("mlp_lo", C.c_double), ("mlp_hi", C.c_double),
+    ]
+
+
+class ICInfo(C.Structure):
+    """mhd2d_ic_info (include/mhd2d.h)."""
+    _fields_ = [
+        ("xa", C.c_double), ("xb", C.c_double), ("ya", C.c_double), ("yb", C.c_double),
+        ("tend", C.c_double), ("M", C.c_double),
+        ("bcR", C.c_int32), ("bcL", C.c_int32), ("bcU", C.c_int32), ("bcD", C.c_int32),
+        ("shock", C.c_int32), ("pad", C.c_int32),
     ]
 
 
@@ -113,6 +127,11 @@ def load_library(path: str = LIB_PATH):
         "mhd2d_trainer_set_data": ([P, fp, fp, C.c_int64], C.c_int),
         "mhd2d_trainer_grad": ([P, fp, C.c_int64, i32, C.c_float, fp, fp], C.c_int),
         "mhd2d_trainer_fit": ([P, fp, i32, i32, C.c_float, C.c_float, C.c_uint64, fp], C.c_int),
+        "mhd2d_checkpoint_save": ([P, C.c_char_p], C.c_int),
+        "mhd2d_checkpoint_load": ([P, C.c_char_p], C.c_int),
+        "mhd2d_ic_defaults": ([i32, C.POINTER(ICInfo)], C.c_int),
+        "mhd2d_calculate_az": ([P], C.c_int),
+        "mhd2d_calculate_l2error": ([P, C.c_double, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -132,6 +151,17 @@ def _fp(a):
 
 def _i64p(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def ic_defaults(ic):
+    """what initN.txt sets for one initial condition: domain (xa, xb, ya, yb),
+    boundary types (bcR, bcL, bcU, bcD), tend, the TVB constant M and the P0
+    flag shock"""
+    info = ICInfo()
+    if load_library().mhd2d_ic_defaults(int(ic), C.byref(info)) != 0:
+        raise MHD2DError(f"unknown initial condition {ic}")
+    return {"domain": (info.xa, info.xb, info.ya, info.yb), "bc": (info.bcR, info.bcL, info.bcU, info.bcD),
+            "tend": info.tend, "M": info.M, "shock": info.shock}
 
 
 def load_indicator_weights(path=WEIGHTS_PATH):
@@ -211,6 +241,20 @@ class Solver:
 
     def set_bc(self):
         self._chk(self.L.mhd2d_set_bc(self.h), "set_bc")
+
+    def calculate_Az(self):
+        self._chk(self.L.mhd2d_calculate_az(self.h), "calculate_Az")
+
+    def calculate_L2error(self, t20):
+        out = np.zeros(8)
+        self._chk(self.L.mhd2d_calculate_l2error(self.h, float(t20), _dp(out)), "calculate_L2error")
+        return out
+
+    def save_checkpoint(self, path):
+        self._chk(self.L.mhd2d_checkpoint_save(self.h, os.fsencode(path)), "checkpoint_save")
+
+    def load_checkpoint(self, path):
+        self._chk(self.L.mhd2d_checkpoint_load(self.h, os.fsencode(path)), "checkpoint_load")
 
     def Lh(self):
         self._chk(self.L.mhd2d_lh(self.h), "Lh")

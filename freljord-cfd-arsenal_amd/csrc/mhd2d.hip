@@ -89,9 +89,10 @@ __global__ void k_scal_reset(DevScal* sc, int which) {
 // calculate_dt.f90:12-24: max |u_n +- c_f| at the cell averages
 template <bool MIX>
 __global__ __launch_bounds__(BS) void k_dt(SRd<MIX> U, Geo g, DevScal* sc) {
-  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
   double ax = 0.0, ay = 0.0;
-  if (t < (long long)g.Nx * g.Ny) {
+  // grid-stride: a bounded grid keeps the same-address atomics below few thousand
+  for (long long t = (long long)blockIdx.x * BS + threadIdx.x; t < (long long)g.Nx * g.Ny;
+       t += (long long)gridDim.x * BS) {
     const int i = 1 + (int)(t % g.Nx), j = 1 + (int)(t / g.Nx);
     double u[8];
 #pragma unroll
@@ -99,10 +100,10 @@ __global__ __launch_bounds__(BS) void k_dt(SRd<MIX> U, Geo g, DevScal* sc) {
     double a1, a2;
     eigenvalueMm<double, 1>(u, a1, a2);
     const double mx = fmax(fabs(a1), fabs(a2));
-    if (mx == mx) ax = mx;     // NaN never exceeds the running max
+    if (mx > ax) ax = mx;      // NaN never exceeds the running max
     eigenvalueMm<double, 2>(u, a1, a2);
     const double my = fmax(fabs(a1), fabs(a2));
-    if (my == my) ay = my;
+    if (my > ay) ay = my;
   }
   __shared__ double sx[BS / 64], sy[BS / 64];
   ax = wave_max(ax);
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(BS) void k_diag(SRd<MIX> U, Geo g, DevScal* sc, dou
   double um = 0.0, pm = 1000000.0, dv = 0.0;
   if (t < (long long)g.Nx * g.Ny) {
     const int i = 1 + (int)(t % g.Nx), j = 1 + (int)(t / g.Nx);
-    const long long c = cix(g, i, j), pl = g.plane;
+    const long long c = cix(g, i, j);
     double m[NPL];
 #pragma unroll
     for (int p = 0; p < NPL; p++) m[p] = U(p, c);
@@ -207,6 +208,7 @@ __global__ void k_div_final(DevScal* sc, const double* partial, long long nblock
 }
 
 __device__ void ic_eval(const ICPar& c, double x, double y, double U[8]) {
+  MHD_NOCONTRACT
   const double pi = 4 * atan(1.0);
   double rho = 1, v1 = 0, v2 = 0, v3 = 0, B1 = 0, B2 = 0, B3 = 0, p = 1;
   const double gam = 5.0 / 3.0;
@@ -245,6 +247,33 @@ __device__ void ic_eval(const ICPar& c, double x, double y, double U[8]) {
     else { rho = 1.0; v1 = 0.0; v2 = 0.0; }
     p = c.par[3];
     B1 = c.par[4];
+  } else if (c.ic == MHD2D_IC_BLAST) {                   // init5.txt:3-18, p0.f90:1-14
+    p = sqrt(x * x + y * y) <= 0.1 ? 1000.0 : 0.1;
+    B1 = 100.0 / sqrt(4 * pi);
+  } else if (c.ic == MHD2D_IC_CLOUD_SHOCK) {            // Cloud_Shock_functions.f90:1-118
+    const double r = sqrt((x - 1.4) * (x - 1.4) + (y - 0.5) * (y - 0.5));
+    if (x <= 1.2) { rho = 3.88968; v3 = -0.05234; p = 14.2641; B3 = 3.9353; }
+    else { rho = r >= 0.18 ? 1.0 : 5.0; v1 = -3.3156; p = 0.04; B3 = 1.0; }
+    B1 = 1.0;
+  } else if (c.ic == MHD2D_IC_CLOUD_SHOCK_2) {          // Cloud_Shock_functions2.f90:1-118
+    const double r = sqrt((x - 0.25) * (x - 0.25) + (y - 0.5) * (y - 0.5));
+    if (x <= 0.05) { rho = 3.88968; v1 = 11.2536; p = 167.345; B2 = 0.21826182; B3 = -0.21826182; }
+    else { rho = r >= 0.15 ? 1.0 : 10.0; p = 1.0; B2 = 0.56418958; B3 = 0.56418958; }
+  } else if (c.ic == MHD2D_IC_MAGNETIC_LOOP) {          // init7.txt:3-18, Loop_functions.f90:1-35
+    const double r = sqrt(x * x + y * y);
+    v1 = 2.0;
+    v2 = 1.0;
+    if (r < 0.3) { B1 = -1e-3 * y / r; B2 = 1e-3 * x / r; }
+  } else if (c.ic == MHD2D_IC_BRIO_WU) {                // BrioWu_functions.f90:1-37
+    rho = x < 0 ? 1.0 : 0.1225; p = x < 0 ? 1.0 : 0.1; B1 = 0.75; B2 = x < 0 ? 1.0 : -1.0;
+  } else if (c.ic == MHD2D_IC_BRIO_WU_Y) {              // BrioWu_functions.f90:40-76
+    rho = y < 0 ? 1.0 : 0.1225; p = y < 0 ? 1.0 : 0.1; B1 = y < 0 ? 1.0 : -1.0; B2 = 0.75;
+  } else if (c.ic == MHD2D_IC_RYU_JONES) {              // RyuJones_functions.f90:1-49
+    rho = x < 0 ? 1.08 : 1.0; p = x < 0 ? 0.95 : 1.0; v1 = x < 0 ? 1.2 : 0.0;
+    B1 = 2.0 / sqrt(4 * pi); B2 = (x < 0 ? 3.6 : 4.0) / sqrt(4 * pi); B3 = 2.0 / sqrt(4 * pi);
+  } else if (c.ic == MHD2D_IC_RYU_JONES_Y) {            // RyuJones_functions.f90:52-100
+    rho = y < 0 ? 1.08 : 1.0; p = y < 0 ? 0.95 : 1.0; v2 = y < 0 ? 1.2 : 0.0;
+    B1 = (y < 0 ? 3.6 : 4.0) / sqrt(4 * pi); B2 = 2.0 / sqrt(4 * pi); B3 = 2.0 / sqrt(4 * pi);
   } else if (c.ic == MHD2D_IC_RIEMANN4) {
     const int q = (x >= c.par[0] ? 0 : 1) + (y >= c.par[1] ? 0 : 2);
     const double* st = &c.par[2 + 6 * q];
@@ -254,7 +283,10 @@ __device__ void ic_eval(const ICPar& c, double x, double y, double U[8]) {
   U[1] = rho * v1;
   U[2] = rho * v2;
   U[3] = rho * v3;
-  U[4] = p / (gam - 1.0) + 0.5 * rho * (v1 * v1 + v2 * v2) + 0.5 * (B1 * B1 + B2 * B2);
+  if (c.flags & IC_FULL_ENERGY)
+    U[4] = p / (gam - 1.0) + 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3) + 0.5 * (B1 * B1 + B2 * B2 + B3 * B3);
+  else
+    U[4] = p / (gam - 1.0) + 0.5 * rho * (v1 * v1 + v2 * v2) + 0.5 * (B1 * B1 + B2 * B2);
   U[5] = B1;
   U[6] = B2;
   U[7] = B3;
@@ -263,6 +295,7 @@ __device__ void ic_eval(const ICPar& c, double x, double y, double U[8]) {
 template <bool MIX>
 __global__ __launch_bounds__(BS) void k_init_cells(SWr<MIX> S, Geo g, double xa, double ya,
                                                    const ICPar* __restrict__ icp) {
+  MHD_NOCONTRACT
   const long long t = (long long)blockIdx.x * BS + threadIdx.x;
   const int W = g.Nx + 2;
   if (t >= (long long)W * (g.Ny + 2)) return;
@@ -274,6 +307,12 @@ __global__ __launch_bounds__(BS) void k_init_cells(SWr<MIX> S, Geo g, double xa,
   }
   const ICPar& ic = *icp;
   const double Xc = xa + (i - 0.5) * g.hx, Yc = ya + (j - 0.5) * g.hy;
+  if (ic.flags & IC_SHOCK) {   // init_data.f90:92-104: P0 state from the centre value
+    double U[8];
+    ic_eval(ic, Xc, Yc, U);
+    for (int p = 0; p < NPL; p++) S.set(p, c, (p < 48 && p % 6 == 0) ? U[p / 6] : 0.0);
+    return;
+  }
   double acc[8][6];
   for (int n = 0; n < 8; n++)
     for (int b = 0; b < 6; b++) acc[n][b] = 0.0;
@@ -296,35 +335,136 @@ __global__ __launch_bounds__(BS) void k_init_cells(SWr<MIX> S, Geo g, double xa,
 
 __global__ __launch_bounds__(BS) void k_init_faces(double* __restrict__ F, Geo g, double xa, double ya,
                                                    const ICPar* __restrict__ icp) {
+  MHD_NOCONTRACT
   const long long t = (long long)blockIdx.x * BS + threadIdx.x;
   const long long nx = (long long)(g.Nx + 1) * g.Ny, ny = (long long)g.Nx * (g.Ny + 1);
   if (t >= nx + ny) return;
   const ICPar& ic = *icp;
   const long long pl = g.plane;
   const double mmE[3] = {1.0, 1.0 / 3.0, 4.0 / 45.0};
+  const bool shock = ic.flags & IC_SHOCK;   // init_data.f90:106-118: face-centre values
   if (t < nx) {
     const int i = (int)(t % (g.Nx + 1)), j = 1 + (int)(t / (g.Nx + 1));
     const double Yc = ya + (j - 0.5) * g.hy;
     double a[3] = {0, 0, 0};
-    for (int q = 0; q < 5; q++) {
+    if (shock) {
       double U[8];
-      ic_eval(ic, xa + i * g.hx, Yc + g.hy1 * lam(q), U);
-      const double e[3] = {1.0, lam(q), P2(lam(q))};
-      for (int k = 0; k < 3; k++) a[k] += 0.5 * wgt(q) * e[k] * U[5];
+      ic_eval(ic, xa + i * g.hx, Yc, U);
+      a[0] = U[5];
+    } else {
+      for (int q = 0; q < 5; q++) {
+        double U[8];
+        ic_eval(ic, xa + i * g.hx, Yc + g.hy1 * lam(q), U);
+        const double e[3] = {1.0, lam(q), P2(lam(q))};
+        for (int k = 0; k < 3; k++) a[k] += 0.5 * wgt(q) * e[k] * U[5];
+      }
+      for (int k = 0; k < 3; k++) a[k] /= mmE[k];
     }
-    for (int k = 0; k < 3; k++) F[k * pl + cix(g, i, j)] = a[k] / mmE[k];
+    for (int k = 0; k < 3; k++) F[k * pl + cix(g, i, j)] = a[k];
   } else {
     const long long u = t - nx;
     const int i = 1 + (int)(u % g.Nx), j = (int)(u / g.Nx);
     const double Xc = xa + (i - 0.5) * g.hx;
     double a[3] = {0, 0, 0};
-    for (int q = 0; q < 5; q++) {
+    if (shock) {
       double U[8];
-      ic_eval(ic, Xc + g.hx1 * lam(q), ya + j * g.hy, U);
-      const double e[3] = {1.0, lam(q), P2(lam(q))};
-      for (int k = 0; k < 3; k++) a[k] += 0.5 * wgt(q) * e[k] * U[6];
+      ic_eval(ic, Xc, ya + j * g.hy, U);
+      a[0] = U[6];
+    } else {
+      for (int q = 0; q < 5; q++) {
+        double U[8];
+        ic_eval(ic, Xc + g.hx1 * lam(q), ya + j * g.hy, U);
+        const double e[3] = {1.0, lam(q), P2(lam(q))};
+        for (int k = 0; k < 3; k++) a[k] += 0.5 * wgt(q) * e[k] * U[6];
+      }
+      for (int k = 0; k < 3; k++) a[k] /= mmE[k];
     }
-    for (int k = 0; k < 3; k++) F[(3 + k) * pl + cix(g, i, j)] = a[k] / mmE[k];
+    for (int k = 0; k < 3; k++) F[(3 + k) * pl + cix(g, i, j)] = a[k];
+  }
+}
+
+// calculate_Az.f90:13-26 with Loop_functions.f90:37-52 (Az of the magnetic loop)
+__device__ double loop_az(double x, double y) {
+  MHD_NOCONTRACT
+  const double r = sqrt(x * x + y * y);
+  return r < 0.3 ? 1e-3 * (0.3 - r) : 0.0;
+}
+
+__global__ __launch_bounds__(BS) void k_calc_az(double* __restrict__ F, Geo g, double xa, double ya) {
+  MHD_NOCONTRACT
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  const long long nx = (long long)(g.Nx + 1) * g.Ny, ny = (long long)g.Nx * (g.Ny + 1);
+  if (t >= nx + ny) return;
+  if (t < nx) {
+    const int i = (int)(t % (g.Nx + 1)), j = 1 + (int)(t / (g.Nx + 1));
+    const double X = xa + i * g.hx;
+    F[cix(g, i, j)] = -(1.0 / g.hy) * (loop_az(X, ya + j * g.hy) - loop_az(X, ya + (j - 1) * g.hy));
+  } else {
+    const long long u = t - nx;
+    const int i = 1 + (int)(u % g.Nx), j = (int)(u / g.Nx);
+    const double Y = ya + j * g.hy;
+    F[3 * g.plane + cix(g, i, j)] = (1.0 / g.hx) * (loop_az(xa + i * g.hx, Y) - loop_az(xa + (i - 1) * g.hx, Y));
+  }
+}
+
+// calculate_L2error.f90:7-33: per-block partial sums of the squared error at
+// the 5x5 Gauss points against the vortex (init1.txt) translated by (t20, t20)
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_l2err(SRd<MIX> U, Geo g, double xa, double ya, double t20,
+                                              const ICPar* __restrict__ icp, double* __restrict__ partial) {
+  MHD_NOCONTRACT
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  double e2[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (t < (long long)g.Nx * g.Ny) {
+    const int i = 1 + (int)(t % g.Nx), j = 1 + (int)(t / g.Nx);
+    const long long c = cix(g, i, j);
+    const double Xc = xa + (i - 0.5) * g.hx, Yc = ya + (j - 0.5) * g.hy;
+    for (int i1 = 0; i1 < 5; i1++)
+      for (int j1 = 0; j1 < 5; j1++) {
+        double ex[8];
+        ic_eval(*icp, Xc + g.hx1 * lam(i1) - t20, Yc + g.hy1 * lam(j1) - t20, ex);
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+          double u = 0.0;
+#pragma unroll
+          for (int b = 0; b < 10; b++) {
+            const int p = PL(b, n);
+            if (p >= 0) u = u + U(p, c) * phi(b, lam(i1), lam(j1));
+          }
+          const double e = u - ex[n];
+          e2[n] = e2[n] + wgt(i1) * wgt(j1) * (e * e);
+        }
+      }
+  }
+  __shared__ double sm[8][BS / 64];
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    const double v = wave_sum(e2[n]);
+    if ((threadIdx.x & 63) == 0) sm[n][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double a = 0.0;
+    for (int w = 0; w < BS / 64; w++) a += sm[threadIdx.x][w];
+    partial[(long long)threadIdx.x * gridDim.x + blockIdx.x] = a;
+  }
+}
+
+// fixed-order sum of the per-block partials, then (L2 hx1 hy1)^(1/2)
+__global__ void k_l2err_final(const double* __restrict__ partial, long long nblocks, double hx1, double hy1,
+                              double* __restrict__ out) {
+  __shared__ double s[BS];
+  for (int n = 0; n < 8; n++) {
+    double a = 0.0;
+    for (long long k = threadIdx.x; k < nblocks; k += BS) a += partial[n * nblocks + k];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = 0.0;
+      for (int k = 0; k < BS; k++) tot += s[k];
+      out[n] = sqrt(tot * hx1 * hy1);
+    }
+    __syncthreads();
   }
 }
 
@@ -617,8 +757,11 @@ int mhd2d_init_data(mhd2d_ctx* ctx, int32_t ic, const double* extra, int32_t nex
     for (int k = 0; k < 8; k++) h.par[k] = k < nextra ? extra[k] : def[k];
   } else if (ic == MHD2D_IC_RIEMANN4) {
     for (int k = 0; k < 26 && k < nextra; k++) h.par[k] = extra[k];
-  } else if (ic != MHD2D_IC_ORSZAG_TANG && ic != MHD2D_IC_VORTEX) {
-    return fail(ctx, -1, "init_data: unknown initial condition %d", ic);
+  } else {
+    mhd2d_ic_info info;
+    if (mhd2d_ic_defaults(ic, &info)) return fail(ctx, -1, "init_data: unknown initial condition %d", ic);
+    if (info.shock) h.flags |= IC_SHOCK;
+    if (ic != MHD2D_IC_ORSZAG_TANG && ic != MHD2D_IC_VORTEX) h.flags |= IC_FULL_ENERGY;
   }
   const Geo& g = ctx->g;
   HIPCHK(ctx, hipMemcpyAsync(ctx->icp, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
@@ -635,6 +778,207 @@ int mhd2d_init_data(mhd2d_ctx* ctx, int32_t ic, const double* extra, int32_t nex
   LAUNCHCHK(ctx);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->lh_done = false;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// checkpoint (format in include/mhd2d.h)
+}  // extern "C"
+namespace {
+
+struct CkHeader {
+  char magic[8];
+  uint32_t version, header_bytes;
+  int32_t Nx, Ny, degree, nplanes, flux_type, precision, bcR, bcL, bcU, bcD;
+  double xa, xb, ya, yb, t, dt;
+  int64_t count;
+  uint64_t payload_bytes;
+  char pad[128 - 8 - 8 - 40 - 48 - 16];
+};
+static_assert(sizeof(CkHeader) == 128, "checkpoint header is 128 bytes");
+
+// CRC-32, reflected polynomial 0xEDB88320 (IEEE 802.3), slicing by 8
+struct Crc32 {
+  uint32_t t[8][256];
+  Crc32() {
+    for (uint32_t i = 0; i < 256; i++) {
+      uint32_t c = i;
+      for (int k = 0; k < 8; k++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+      t[0][i] = c;
+    }
+    for (uint32_t i = 0; i < 256; i++)
+      for (int s = 1; s < 8; s++) t[s][i] = (t[s - 1][i] >> 8) ^ t[0][t[s - 1][i] & 0xFF];
+  }
+  uint32_t update(uint32_t crc, const void* data, size_t n) const {
+    const unsigned char* p = (const unsigned char*)data;
+    uint32_t c = ~crc;
+    while (n >= 8) {
+      uint32_t lo, hi;
+      memcpy(&lo, p, 4);
+      memcpy(&hi, p + 4, 4);
+      lo ^= c;
+      c = t[7][lo & 0xFF] ^ t[6][(lo >> 8) & 0xFF] ^ t[5][(lo >> 16) & 0xFF] ^ t[4][lo >> 24] ^ t[3][hi & 0xFF] ^
+          t[2][(hi >> 8) & 0xFF] ^ t[1][(hi >> 16) & 0xFF] ^ t[0][hi >> 24];
+      p += 8;
+      n -= 8;
+    }
+    while (n--) c = t[0][(c ^ *p++) & 0xFF] ^ (c >> 8);
+    return ~c;
+  }
+};
+const Crc32& crc32_tables() {
+  static const Crc32 tab;
+  return tab;
+}
+
+// active planes in device plane order (the order PL() assigns)
+void plane_list(int (&bn)[52][2]) {
+  for (int n = 0; n < 8; n++)
+    for (int b = 0; b < 10; b++) {
+      const int p = PL(b, n);
+      if (p >= 0) { bn[p][0] = b; bn[p][1] = n; }
+    }
+}
+
+}  // namespace
+extern "C" {
+
+int mhd2d_checkpoint_save(mhd2d_ctx* ctx, const char* path) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if (!path) return fail(ctx, -1, "checkpoint_save: null path");
+  const Geo& g = ctx->g;
+  const size_t W = g.Nx + 2, H = g.Ny + 2;
+  const size_t nbx = (size_t)(g.Nx + 1) * g.Ny * 3, nby = (size_t)g.Nx * (g.Ny + 1) * 3;
+  std::vector<double> uh(W * H * 80), pay(52 * W * H + nbx + nby);
+  if ((rc = mhd2d_download(ctx, uh.data(), pay.data() + 52 * W * H, pay.data() + 52 * W * H + nbx))) return rc;
+  int bn[52][2];
+  plane_list(bn);
+  for (int p = 0; p < 52; p++)
+    memcpy(pay.data() + p * W * H, uh.data() + W * H * (bn[p][0] + 10 * bn[p][1]), W * H * sizeof(double));
+  CkHeader h;
+  memset(&h, 0, sizeof h);
+  memcpy(h.magic, "MHD2DCK1", 8);
+  h.version = 1;
+  h.header_bytes = sizeof h;
+  h.Nx = g.Nx; h.Ny = g.Ny; h.degree = 2; h.nplanes = 52;
+  h.flux_type = ctx->cfg.flux_type; h.precision = ctx->cfg.precision;
+  h.bcR = g.bcR; h.bcL = g.bcL; h.bcU = g.bcU; h.bcD = g.bcD;
+  h.xa = ctx->cfg.xa; h.xb = ctx->cfg.xb; h.ya = ctx->cfg.ya; h.yb = ctx->cfg.yb;
+  if ((rc = mhd2d_get_time(ctx, &h.t, &h.dt, &h.count))) return rc;
+  h.payload_bytes = pay.size() * sizeof(double);
+  const Crc32& C = crc32_tables();
+  uint32_t crc = C.update(0, &h, sizeof h);
+  crc = C.update(crc, pay.data(), h.payload_bytes);
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(ctx, -4, "checkpoint_save: cannot open %s", path);
+  bool ok = fwrite(&h, sizeof h, 1, f) == 1 && fwrite(pay.data(), 1, h.payload_bytes, f) == h.payload_bytes &&
+            fwrite(&crc, 4, 1, f) == 1;
+  ok = (fclose(f) == 0) && ok;
+  if (!ok) return fail(ctx, -4, "checkpoint_save: write to %s failed", path);
+  return 0;
+}
+
+int mhd2d_checkpoint_load(mhd2d_ctx* ctx, const char* path) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if (!path) return fail(ctx, -1, "checkpoint_load: null path");
+  const Geo& g = ctx->g;
+  const size_t W = g.Nx + 2, H = g.Ny + 2;
+  const size_t nbx = (size_t)(g.Nx + 1) * g.Ny * 3, nby = (size_t)g.Nx * (g.Ny + 1) * 3;
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(ctx, -4, "checkpoint_load: cannot open %s", path);
+  CkHeader h;
+  std::vector<double> pay;
+  uint32_t crc_file = 0;
+  const char* why = nullptr;
+  if (fread(&h, sizeof h, 1, f) != 1) why = "short header";
+  else if (memcmp(h.magic, "MHD2DCK1", 8) || h.version != 1 || h.header_bytes != sizeof h) why = "not a version-1 checkpoint";
+  else if (h.Nx != g.Nx || h.Ny != g.Ny || h.degree != 2 || h.nplanes != 52) why = "grid differs from the context";
+  else if (h.payload_bytes != (52 * W * H + nbx + nby) * sizeof(double)) why = "payload size differs";
+  if (!why) {
+    pay.resize(h.payload_bytes / sizeof(double));
+    if (fread(pay.data(), 1, h.payload_bytes, f) != h.payload_bytes || fread(&crc_file, 4, 1, f) != 1)
+      why = "truncated file";
+  }
+  fclose(f);
+  if (why) return fail(ctx, -4, "checkpoint_load: %s: %s", path, why);
+  const Crc32& C = crc32_tables();
+  const uint32_t crc = C.update(C.update(0, &h, sizeof h), pay.data(), h.payload_bytes);
+  if (crc != crc_file) return fail(ctx, -4, "checkpoint_load: %s: CRC-32 mismatch (file %08x, data %08x)", path, crc_file, crc);
+  std::vector<double> uh(W * H * 80, 0.0);
+  int bn[52][2];
+  plane_list(bn);
+  for (int p = 0; p < 52; p++)
+    memcpy(uh.data() + W * H * (bn[p][0] + 10 * bn[p][1]), pay.data() + p * W * H, W * H * sizeof(double));
+  if ((rc = mhd2d_upload(ctx, uh.data(), pay.data() + 52 * W * H, pay.data() + 52 * W * H + nbx))) return rc;
+  return mhd2d_set_time(ctx, h.t, h.count);
+}
+
+int mhd2d_ic_defaults(int32_t ic, mhd2d_ic_info* out) {
+  if (!out) return -1;
+  const double pi = 4 * atan(1.0);
+  struct Row { int ic; double xa, xb, ya, yb, tend, M; int bc, shock; };
+  // initN.txt: domain (xa..yb), tend, M, boundary type (all four sides) and shock
+  const Row rows[] = {
+      {MHD2D_IC_VORTEX, -10, 10, -10, 10, 0, 1000000, 1, 0},                  // init1.txt:41-53
+      {MHD2D_IC_ORSZAG_TANG, 0, 2 * pi, 0, 2 * pi, 3, 10, 1, 0},               // init3.txt
+      {MHD2D_IC_ROTOR, 0, 1, 0, 1, 0, 10, 1, 0},                               // init4.txt
+      {MHD2D_IC_BLAST, -0.5, 0.5, -0.5, 0.5, 0.01, 10, 2, 1},                  // init5.txt
+      {MHD2D_IC_CLOUD_SHOCK, 0, 2, 0, 1, 0.6, 10, 2, 1},                       // init6.txt
+      {MHD2D_IC_CLOUD_SHOCK_2, 0, 1, 0, 1, 0.06, 10, 2, 1},                    // init6b.txt
+      {MHD2D_IC_MAGNETIC_LOOP, -1, 1, -0.5, 0.5, 2, 10, 1, 0},                 // init7.txt
+      {MHD2D_IC_BRIO_WU, -0.5, 0.8, 0, 0.1, 0, 1, 2, 0},                       // init8.txt
+      {MHD2D_IC_BRIO_WU_Y, 0, 0.1, -0.5, 0.8, 0.2, 1, 2, 1},                   // init8b.txt
+      {MHD2D_IC_RYU_JONES, -0.5, 0.5, 0, 0.1, 0.2, 1, 2, 0},                   // init9.txt
+      {MHD2D_IC_RYU_JONES_Y, 0, 0.1, -0.5, 0.5, 0.2, 1, 2, 0},                 // init9b.txt
+      {MHD2D_IC_OT_PERTURBED, 0, 2 * pi, 0, 2 * pi, 3, 50, 1, 0},
+      {MHD2D_IC_RIEMANN4, 0, 1, 0, 1, 0.2, 10, 2, 0},
+  };
+  for (const Row& r : rows)
+    if (r.ic == ic) {
+      *out = mhd2d_ic_info{r.xa, r.xb, r.ya, r.yb, r.tend, r.M, r.bc, r.bc, r.bc, r.bc, r.shock, 0};
+      return 0;
+    }
+  return -1;
+}
+
+int mhd2d_calculate_az(mhd2d_ctx* ctx) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  const Geo& g = ctx->g;
+  const long long nf = (long long)(g.Nx + 1) * g.Ny + (long long)g.Nx * (g.Ny + 1);
+  hipLaunchKernelGGL(k_calc_az, dim3(nblk(nf)), dim3(BS), 0, ctx->stream, ctx->F, g, ctx->cfg.xa, ctx->cfg.ya);
+  LAUNCHCHK(ctx);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->lh_done = false;
+  return 0;
+}
+
+int mhd2d_calculate_l2error(mhd2d_ctx* ctx, double t20, double L2[8]) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if (!L2) return fail(ctx, -1, "calculate_l2error: null output");
+  const Geo& g = ctx->g;
+  ICPar h{};
+  h.ic = MHD2D_IC_VORTEX;
+  const long long nb = ctx->nblk_cells;
+  double* buf = nullptr;
+  HIPCHK(ctx, hipMalloc((void**)&buf, (size_t)(8 * nb + 8) * sizeof(double)));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->icp, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
+  rc = with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_l2err<MIX>), dim3((unsigned)nb), dim3(BS), 0, ctx->stream, SRd<MIX>{ctx->S, g.plane}, g,
+                       ctx->cfg.xa, ctx->cfg.ya, t20, (const ICPar*)ctx->icp, buf);
+    return 0;
+  });
+  hipLaunchKernelGGL(k_l2err_final, dim3(1), dim3(BS), 0, ctx->stream, (const double*)buf, nb, g.hx1, g.hy1,
+                     buf + 8 * nb);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(L2, buf + 8 * nb, 8 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return fail(ctx, -2, "calculate_l2error: %s", hipGetErrorString(e));
   return 0;
 }
 
@@ -747,8 +1091,8 @@ static int launch_dt(mhd2d_ctx* ctx, int advance) {
   hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, 0);
   with_mix(ctx, [&](auto Mc) -> int {
     constexpr bool MIX = decltype(Mc)::value;
-    hipLaunchKernelGGL((k_dt<MIX>), dim3(ctx->nblk_cells), dim3(BS), 0, ctx->stream, SRd<MIX>{ctx->S, g.plane}, g,
-                       ctx->sc);
+    hipLaunchKernelGGL((k_dt<MIX>), dim3((unsigned)std::min<long long>(ctx->nblk_cells, 2048)), dim3(BS), 0,
+                       ctx->stream, SRd<MIX>{ctx->S, g.plane}, g, ctx->sc);
     return 0;
   });
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, g.hx, g.hy, advance);

@@ -89,10 +89,13 @@ __device__ __forceinline__ void trace_at(const SV& U, long long c, int q, T out[
 // initial conditions (init_data.f90:21-121; init1/3/4.txt; Rotor_functions.f90)
 struct ICPar {
   int ic;
-  int pad;
+  int flags;            // IC_FULL_ENERGY | IC_SHOCK
   double par[32];
   double kx[16], ky[16], phu[16], phv[16];
 };
+
+// kinetic and magnetic energy include w and Bz (init5-9b); P0 data (shock = 1)
+constexpr int IC_FULL_ENERGY = 1, IC_SHOCK = 2;
 
 // the indicator MLP 60-64-64-1: parameter vector layout (FP32)
 constexpr int MLP_NIN = 60;

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(BS) void k_limit(SRd<MIX> in, SWr<MIX> out,
   const int W = g.Nx + 2;
   if (t >= (long long)W * (g.Ny + 2)) return;
   const int i = (int)(t % W), j = (int)(t / W);
-  const long long c = cix(g, i, j), pl = g.plane;
+  const long long c = cix(g, i, j);
   const bool interior = i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny;
   if (!interior) {
 #pragma unroll 4
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(BS) void k_detect(SRd<MIX> U, int* __restrict__ tro
 template <class SV>
 __device__ __forceinline__ int tvb_cell(const SV& in, const Geo& g, int i, int j, double M,
                                         double Rm[8], double Lm[8], double Um[8], double Dm[8]) {
-  const long long pl = g.plane, c = cix(g, i, j);
+  const long long c = cix(g, i, j);
   int change = 0;
   double U[8];
 #pragma unroll

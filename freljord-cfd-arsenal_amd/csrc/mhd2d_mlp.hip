@@ -45,7 +45,6 @@ constexpr int XP = 68;          // LDS row pitch (bf16) of a 64-wide activation 
 template <class SV>
 __device__ __forceinline__ void mlp_features(const SV& U, const Geo& g, int i, int j,
                                              double Mh2, float f[MLP_NIN]) {
-  const long long pl = g.plane;
   const long long cs[5] = {cix(g, i, j), cix(g, i + 1, j), cix(g, i - 1, j), cix(g, i, j + 1), cix(g, i, j - 1)};
   constexpr int dm[4] = {1, 2, 3, 5};
   double r[5], e[5], mu[5], mv[5];
@@ -93,7 +92,6 @@ __device__ __forceinline__ void band_append(bool band, int t, int* __restrict__ 
 template <class SV>
 __device__ __forceinline__ void mlp_features_f32(const SV& U, const Geo& g, int i, int j,
                                                  float l10Mh2, float l10h, float f[MLP_NIN]) {
-  const long long pl = g.plane;
   const long long cs[5] = {cix(g, i, j), cix(g, i + 1, j), cix(g, i - 1, j), cix(g, i, j + 1), cix(g, i, j - 1)};
   constexpr int dm[4] = {1, 2, 3, 5};
   float r[5], e[5], mu[5], mv[5];

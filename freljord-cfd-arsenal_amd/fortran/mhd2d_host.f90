@@ -30,7 +30,16 @@ module mhd2d_capi
                                    MHD2D_IND_MLP = 3
   integer(c_int32_t), parameter :: MHD2D_FP64 = 0, MHD2D_MIXED = 1
   integer(c_int32_t), parameter :: MHD2D_IC_VORTEX = 1, MHD2D_IC_ORSZAG_TANG = 3, MHD2D_IC_ROTOR = 4, &
+                                   MHD2D_IC_BLAST = 5, MHD2D_IC_CLOUD_SHOCK = 6, MHD2D_IC_CLOUD_SHOCK_2 = 61, &
+                                   MHD2D_IC_MAGNETIC_LOOP = 7, MHD2D_IC_BRIO_WU = 8, MHD2D_IC_BRIO_WU_Y = 81, &
+                                   MHD2D_IC_RYU_JONES = 9, MHD2D_IC_RYU_JONES_Y = 91, &
                                    MHD2D_IC_OT_PERTURBED = 103, MHD2D_IC_RIEMANN4 = 104
+
+  ! struct mhd2d_ic_info (include/mhd2d.h)
+  type, bind(C) :: mhd2d_ic_info
+    real(c_double) :: xa, xb, ya, yb, tend, M
+    integer(c_int32_t) :: bcR, bcL, bcU, bcD, shock, pad
+  end type mhd2d_ic_info
 
   ! struct mhd2d_config (include/mhd2d.h)
   type, bind(C) :: mhd2d_config
@@ -139,6 +148,31 @@ module mhd2d_capi
       real(c_double), value :: t
       integer(c_int64_t), value :: count
     end function
+    integer(c_int) function mhd2d_ic_defaults(ic, info) bind(C, name='mhd2d_ic_defaults')
+      import :: c_int, c_int32_t, mhd2d_ic_info
+      integer(c_int32_t), value :: ic
+      type(mhd2d_ic_info), intent(out) :: info
+    end function
+    integer(c_int) function mhd2d_calculate_az(ctx) bind(C, name='mhd2d_calculate_az')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function mhd2d_calculate_l2error(ctx, t20, L2) bind(C, name='mhd2d_calculate_l2error')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t20
+      real(c_double), intent(out) :: L2(8)
+    end function
+    integer(c_int) function mhd2d_checkpoint_save(ctx, path) bind(C, name='mhd2d_checkpoint_save')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+    integer(c_int) function mhd2d_checkpoint_load(ctx, path) bind(C, name='mhd2d_checkpoint_load')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+    end function
   end interface
 
 contains
@@ -175,6 +209,8 @@ module mhd2d_common
   real(c_double), allocatable :: uh(:, :, :, :), Bx(:, :, :), By(:, :, :)
   ! com.txt:20 /time/ and the diagnostics of com.txt:22,43,54
   real(c_double) :: t = 0, dt = 0, tend = 0, umax = 0, totaldiv = 0, pmin = 0
+  ! com.txt:20 t20 and com.txt:24 L2 (calculate_L2error)
+  real(c_double) :: t20 = 0, L2(8) = 0
   integer(c_int64_t) :: count = 0, nflag = 0
   logical :: verbose = .true.
 
@@ -298,6 +334,33 @@ subroutine init_data(ic, extra, nextra)
   real(c_double), intent(in) :: extra(*)
   call check(mhd2d_init_data(ctx, int(ic, c_int32_t), extra, int(nextra, c_int32_t)), 'init_data')
 end subroutine init_data
+
+! calculate_Az.f90:1-26 (magnetic loop: mode 1 of Bx, By from Az)
+subroutine calculate_Az()
+  use mhd2d_common
+  call check(mhd2d_calculate_az(ctx), 'calculate_Az')
+end subroutine calculate_Az
+
+! calculate_L2error.f90:1-37: L2 against the vortex translated by t20
+subroutine calculate_L2error()
+  use mhd2d_common
+  call check(mhd2d_calculate_l2error(ctx, t20, L2), 'calculate_L2error')
+  if (verbose) print *, L2
+end subroutine calculate_L2error
+
+! binary checkpoint incl. the face fields and (t, count) (include/mhd2d.h)
+subroutine save_checkpoint(path)
+  use mhd2d_common
+  character(len=*), intent(in) :: path
+  call check(mhd2d_checkpoint_save(ctx, trim(path) // c_null_char), 'save_checkpoint')
+end subroutine save_checkpoint
+
+subroutine load_checkpoint(path)
+  use mhd2d_common
+  character(len=*), intent(in) :: path
+  call check(mhd2d_checkpoint_load(ctx, trim(path) // c_null_char), 'load_checkpoint')
+  call sync_time()
+end subroutine load_checkpoint
 
 ! RK3.f90:5-174 without frame output.  t, dt and count stay on the device
 ! inside the loop (mhd2d_step clips dt to tend and stops a step that

@@ -8,6 +8,11 @@
 ! then writes what main.f90 writes (save_XY, save_solution) when qtxt is
 ! true and, when out is set, the final uh, Bx, By as raw little-endian
 ! float64 in the reference layouts, preceded by 4 float64: t, count, Nx, Ny.
+! Without xa < xb the domain, boundaries and M come from the initial
+! condition's initN.txt (mhd2d_ic_defaults).  restart = <file> continues from
+! a checkpoint instead of projecting the initial condition (no prologue);
+! checkpoint = <file> writes one at the end; l2err calls calculate_L2error
+! with t20 = tend if tend <= 5, else 0 (main.f90:13-19).
 
 program mhd2d_run
   use mhd2d_common
@@ -17,11 +22,12 @@ program mhd2d_run
   real(c_double) :: xa = 0, xb = 0, ya = 0, yb = 0, M = 10, CFL = 0.15d0, Ck = 1, mlp_lo = 0.2d0, mlp_hi = 0.8d0
   real(c_double) :: tend_ = 0.1d0, extra(64) = 0
   character(len=16) :: mode = 'fused'
-  character(len=512) :: out = '', nml = ''
-  logical :: qtxt = .false., quiet = .false.
+  character(len=512) :: out = '', nml = '', restart = '', checkpoint = ''
+  logical :: qtxt = .false., quiet = .false., l2err = .false.
   type(mhd2d_config) :: c
+  type(mhd2d_ic_info) :: info
   namelist /run/ Nx_, Ny_, flux_type, ic, bc, pp, indicator, precision, max_steps, xa, xb, ya, yb, M, CFL, Ck, &
-                 mlp_lo, mlp_hi, tend_, extra, nextra, mode, out, qtxt, quiet
+                 mlp_lo, mlp_hi, tend_, extra, nextra, mode, out, qtxt, quiet, restart, checkpoint, l2err
   interface
     subroutine init_data(ic, extra, nextra)
       import :: c_double
@@ -30,6 +36,12 @@ program mhd2d_run
     end subroutine
     subroutine RK3_subroutines(max_steps)
       integer, intent(in) :: max_steps
+    end subroutine
+    subroutine load_checkpoint(path)
+      character(len=*), intent(in) :: path
+    end subroutine
+    subroutine save_checkpoint(path)
+      character(len=*), intent(in) :: path
     end subroutine
   end interface
 
@@ -41,11 +53,17 @@ program mhd2d_run
   open(newunit=u, file=trim(nml), status='old', action='read')
   read(u, nml=run)
   close(u)
-  if (xb <= xa) then       ! default domain [0, 2 pi]^2 (init3.txt)
-    xa = 0
-    xb = 8 * atan(1d0)
-    ya = 0
-    yb = xb
+  if (xb <= xa) then       ! the initial condition's own domain, boundaries and M
+    if (mhd2d_ic_defaults(int(ic, c_int32_t), info) /= 0) then
+      write(0, '(a,i0)') 'unknown initial condition ', ic
+      error stop 2
+    end if
+    xa = info%xa
+    xb = info%xb
+    ya = info%ya
+    yb = info%yb
+    bc = info%bcR
+    M = info%M
   end if
 
   c = mhd2d_config(Nx_, Ny_, flux_type, bc, bc, bc, bc, pp, indicator, precision, 0, &
@@ -54,8 +72,18 @@ program mhd2d_run
   tend = tend_
   verbose = .not. quiet
 
-  call init_data(ic, extra, nextra)
-  if (trim(mode) == 'subroutines') then
+  if (len_trim(restart) > 0) then
+    call load_checkpoint(restart)
+    do while (t < tend .and. (max_steps <= 0 .or. count < max_steps))
+      call check(mhd2d_step(ctx, tend, t), 'RK3 step')
+      call sync_time()
+    end do
+  else
+    call init_data(ic, extra, nextra)
+  end if
+  if (len_trim(restart) > 0) then
+    continue
+  else if (trim(mode) == 'subroutines') then
     call RK3_subroutines(max_steps)
   else if (max_steps > 0) then
     ! fused step, bounded step count (same loop test as RK3)
@@ -79,6 +107,12 @@ program mhd2d_run
     call save_XY()
     call save_solution()
   end if
+  if (l2err) then
+    t20 = 0
+    if (tend <= 5) t20 = tend
+    call calculate_L2error()
+  end if
+  if (len_trim(checkpoint) > 0) call save_checkpoint(checkpoint)
   if (len_trim(out) > 0) then
     call gpu_download()
     open(newunit=u, file=trim(out), access='stream', form='unformatted', status='replace')

@@ -43,9 +43,25 @@ enum {
   MHD2D_IC_VORTEX = 1,           /* init1.txt                                */
   MHD2D_IC_ORSZAG_TANG = 3,      /* init3.txt                                */
   MHD2D_IC_ROTOR = 4,            /* init4.txt + Rotor_functions.f90          */
+  MHD2D_IC_BLAST = 5,            /* init5.txt + p0.f90 (shock = 1: P0 data)  */
+  MHD2D_IC_CLOUD_SHOCK = 6,      /* init6.txt + Cloud_Shock_functions.f90    */
+  MHD2D_IC_CLOUD_SHOCK_2 = 61,   /* init6b.txt + Cloud_Shock_functions2.f90  */
+  MHD2D_IC_MAGNETIC_LOOP = 7,    /* init7.txt + Loop_functions.f90           */
+  MHD2D_IC_BRIO_WU = 8,          /* init8.txt + BrioWu_functions.f90         */
+  MHD2D_IC_BRIO_WU_Y = 81,       /* init8b.txt (shock = 1)                   */
+  MHD2D_IC_RYU_JONES = 9,        /* init9.txt + RyuJones_functions.f90       */
+  MHD2D_IC_RYU_JONES_Y = 91,     /* init9b.txt                               */
   MHD2D_IC_OT_PERTURBED = 103,   /* Orszag-Tang + seeded 16-mode velocity    */
   MHD2D_IC_RIEMANN4 = 104        /* four constant quadrants                  */
 };
+/* what initN.txt sets besides the fields (domain, boundaries, TVB M, tend,
+   the P0 flag `shock` of init_data.f90:92) */
+typedef struct mhd2d_ic_info {
+  double xa, xb, ya, yb, tend, M;
+  int32_t bcR, bcL, bcU, bcD;
+  int32_t shock;
+  int32_t pad;
+} mhd2d_ic_info;
 /* arrays readable with mhd2d_debug_get */
 enum { MHD2D_DU = 0, MHD2D_DEZ1 = 1, MHD2D_DEZ2 = 2, MHD2D_EZVERTEX = 3, MHD2D_TROUBLE = 4 };
 
@@ -78,6 +94,31 @@ int mhd2d_download(mhd2d_ctx *ctx, double *uh, double *Bx, double *By);
    extra: IC parameters (perturbed OT: amplitude, seed; rotor: r0, r1, spin,
    p, Bx, rho_in, xc, yc; Riemann4: x0, y0, 4 x (rho,u,v,p,Bx,By)). */
 int mhd2d_init_data(mhd2d_ctx *ctx, int32_t ic, const double *extra, int32_t nextra);
+/* Binary checkpoint of the state the time loop carries (RK3.f90:57-172):
+   uh, the face fields Bx/By and (t, count), so that a run restarts from
+   it bit for bit (the reference writes only uh and has no restart).
+   Little-endian file = 128-byte header, payload, CRC-32 trailer:
+     header  "MHD2DCK1", u32 version 1, u32 header bytes 128, i32 Nx, Ny,
+             polynomial degree 2, planes 52, flux_type, precision, bcR, bcL,
+             bcU, bcD, then f64 xa, xb, ya, yb, t, dt, i64 count,
+             u64 payload bytes, zero padding
+     payload the 52 active (mode, variable) planes of uh(0:Nx+1, 0:Ny+1, :, :)
+             in the order mode 1-6 of variables 1-8, then (7, Bx), (9, Bx),
+             (8, By), (10, By), each plane i fastest; then Bx(0:Nx, 1:Ny, 3)
+             and By(1:Nx, 0:Ny, 3) in Fortran order (all f64)
+     trailer u32 CRC-32 (IEEE 802.3, as zlib.crc32) of header + payload
+   load checks magic, version, Nx, Ny, size and CRC before touching the
+   context, then uploads the state and sets (t, count). */
+int mhd2d_checkpoint_save(mhd2d_ctx *ctx, const char *path);
+int mhd2d_checkpoint_load(mhd2d_ctx *ctx, const char *path);
+/* defaults of one initial condition (initN.txt); -1 for an unknown ic */
+int mhd2d_ic_defaults(int32_t ic, mhd2d_ic_info *out);
+/* calculate_Az.f90:1-26: mode 1 of Bx and By from the magnetic loop's
+   vector potential (Loop_functions.f90:37-52), modes 2-3 untouched */
+int mhd2d_calculate_az(mhd2d_ctx *ctx);
+/* calculate_L2error.f90:1-37: L2 error of each conserved variable against
+   the smooth vortex (init1.txt) translated by (t20, t20) */
+int mhd2d_calculate_l2error(mhd2d_ctx *ctx, double t20, double L2[8]);
 
 /* the reference's subroutines */
 int mhd2d_set_bc(mhd2d_ctx *ctx);                        /* set_bc.f90:5-27          */

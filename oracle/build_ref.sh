@@ -9,9 +9,13 @@
 # compile time, so each variant applies these configuration edits to the
 # scratch copy (the same edits the author makes by hand):
 #   P1 com.txt:1            Nx, Ny, flux_type; flash = 0 (no frame files)
-#   P2 init_data.f90:19     one object per initial condition (init3 = Orszag-
-#                           Tang, init1 = vortex, init4 = rotor), subroutine
-#                           renamed init_data_3 / _1 / _4
+#   P2 init_data.f90:19     one object per initial condition (init1 vortex,
+#                           init3 Orszag-Tang, init4 rotor, init5 blast,
+#                           init6/6b cloud-shock, init7 magnetic loop,
+#                           init8/8b Brio-Wu, init9/9b Ryu-Jones), subroutine
+#                           renamed init_data_<N>.  init0 and init2 define six
+#                           conserved variables (an older equation set) and do
+#                           not compile with this init_data
 #   P5 TVB_Limiter_P2.f90:123  'call pp_Limiter' enabled (variants *_pp)
 #   P7 HLL_Flux_2D.f90:8    EzRU..EzLD are read uninitialised at :176-177;
 #                           they are set to the corner fields v*B1 - u*B2
@@ -58,7 +62,9 @@ CK=${CK:-1}
 SRCS="get_GLP get_basis set_bc Lh LhEz eigenvalueMm HLL_Flux LF_Flux HLLC_Flux HLLD_Flux
 HLL_Flux_2D LF_Flux_2D HLLC_Flux_2D HLLD_Flux_2D HLL1 MHD_flux calculate_Ez TVB_Limiter_P2
 minmod minmodB eigenmatrix pp_Limiter pressure div_free_Balsara calculate_dt calculate_umax
-calculate_pmin calculate_totaldiv RK3 Rotor_functions KXRCF_Detector"
+calculate_pmin calculate_totaldiv RK3 Rotor_functions KXRCF_Detector p0 Cloud_Shock_functions
+Cloud_Shock_functions2 Loop_functions BrioWu_functions RyuJones_functions calculate_Az calculate_L2error"
+ICS="3 1 4 5 6 6b 7 8 8b 9 9b"
 
 build_one() {
   local name=$1 nx=$2 ny=$3 ft=$4 pp=$5 omp=$6 kx=$7
@@ -69,7 +75,7 @@ build_one() {
   # P1
   sed -i "1s/Nx = [0-9]*, Ny = [0-9]*/Nx = $nx, Ny = $ny/; 1s/flux_type = [0-9]*/flux_type = $ft/; 1s/flash = [0-9]*/flash = 0/" "$w/com.txt"
   # P2
-  for ic in 3 1 4; do
+  for ic in $ICS; do
     sed -e "s/include 'init6.txt'/include 'init$ic.txt'/" \
         -e "s/subroutine init_data\b/subroutine init_data_$ic/" \
         "$w/init_data.f90" > "$w/init_data_$ic.f90"
@@ -101,7 +107,8 @@ build_one() {
   if [ $((nx * ny)) -gt 250000 ]; then flags="$flags -mcmodel=medium"; ldflags="$ldflags -mcmodel=medium -fuse-ld=bfd"; fi
   (
     cd "$w"
-    units="$SRCS init_data_3 init_data_1 init_data_4 ref_driver"
+    units="$SRCS ref_driver"
+    for ic in $ICS; do units="$units init_data_$ic"; done
     objs=""
     for f in $units; do objs="$objs $f.o"; done
     for f in $units; do echo "$FC $flags -c $f.f90"; done | xargs -P "$JOBS" -I{} sh -c "{}"

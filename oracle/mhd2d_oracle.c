@@ -819,6 +819,8 @@ void orc_set_time(orc_state *s, double t, double dt) { s->t = t; s->dt = dt; }
 
 typedef struct {
   int ic;
+  int full_energy;   /* init5-9: kinetic and magnetic energy include w and Bz */
+  int shock;         /* init5, 6, 6b, 8b: shock = 1 (init_data.f90:92-121) */
   double par[32];
   double kx[16], ky[16], phu[16], phv[16];
 } icdef_t;
@@ -877,6 +879,46 @@ static void ic_eval(const icdef_t *c, double x, double y, double U[8]) {
       B1 = c->par[4];
       break;
     }
+    case ORC_IC_BLAST: { /* init5.txt:3-18, p0.f90:1-14 */
+      double r = pow(x * x + y * y, 0.5);
+      p = r <= 0.1 ? 1000.0 : 0.1;
+      B1 = 100.0 / pow(4 * pi, 0.5);
+      break;
+    }
+    case ORC_IC_CLOUD_SHOCK: { /* init6.txt:3-18, Cloud_Shock_functions.f90:1-118 */
+      double r = pow((x - 1.4) * (x - 1.4) + (y - 0.5) * (y - 0.5), 0.5);
+      if (x <= 1.2) { rho = 3.88968; v3 = -0.05234; p = 14.2641; B3 = 3.9353; }
+      else { rho = r >= 0.18 ? 1.0 : 5.0; v1 = -3.3156; p = 0.04; B3 = 1.0; }
+      B1 = 1.0;
+      break;
+    }
+    case ORC_IC_CLOUD_SHOCK_2: { /* init6b.txt:3-18, Cloud_Shock_functions2.f90:1-118 */
+      double r = pow((x - 0.25) * (x - 0.25) + (y - 0.5) * (y - 0.5), 0.5);
+      if (x <= 0.05) { rho = 3.88968; v1 = 11.2536; p = 167.345; B2 = 0.21826182; B3 = -0.21826182; }
+      else { rho = r >= 0.15 ? 1.0 : 10.0; p = 1.0; B2 = 0.56418958; B3 = 0.56418958; }
+      break;
+    }
+    case ORC_IC_MAGNETIC_LOOP: { /* init7.txt:3-18, Loop_functions.f90:1-37 */
+      double r = pow(x * x + y * y, 0.5);
+      v1 = 2.0;
+      v2 = 1.0;
+      if (r < 0.3) { B1 = -1e-3 * y / r; B2 = 1e-3 * x / r; }
+      break;
+    }
+    case ORC_IC_BRIO_WU: /* init8.txt:3-18, BrioWu_functions.f90:1-37 */
+      rho = x < 0 ? 1.0 : 0.1225; p = x < 0 ? 1.0 : 0.1; B1 = 0.75; B2 = x < 0 ? 1.0 : -1.0;
+      break;
+    case ORC_IC_BRIO_WU_Y: /* init8b.txt:3-18, BrioWu_functions.f90:40-76 */
+      rho = y < 0 ? 1.0 : 0.1225; p = y < 0 ? 1.0 : 0.1; B1 = y < 0 ? 1.0 : -1.0; B2 = 0.75;
+      break;
+    case ORC_IC_RYU_JONES: /* init9.txt:3-18, RyuJones_functions.f90:1-49 */
+      rho = x < 0 ? 1.08 : 1.0; p = x < 0 ? 0.95 : 1.0; v1 = x < 0 ? 1.2 : 0.0;
+      B1 = 2.0 / pow(4 * pi, 0.5); B2 = (x < 0 ? 3.6 : 4.0) / pow(4 * pi, 0.5); B3 = 2.0 / pow(4 * pi, 0.5);
+      break;
+    case ORC_IC_RYU_JONES_Y: /* init9b.txt:3-18, RyuJones_functions.f90:52-100 */
+      rho = y < 0 ? 1.08 : 1.0; p = y < 0 ? 0.95 : 1.0; v2 = y < 0 ? 1.2 : 0.0;
+      B1 = (y < 0 ? 3.6 : 4.0) / pow(4 * pi, 0.5); B2 = 2.0 / pow(4 * pi, 0.5); B3 = 2.0 / pow(4 * pi, 0.5);
+      break;
     case ORC_IC_RIEMANN4: { /* piecewise constant; par: x0, y0, then 4 x (rho,u,v,p,Bx,By) for NE, NW, SE, SW */
       int q = (x >= c->par[0] ? 0 : 1) + (y >= c->par[1] ? 0 : 2);
       const double *st = &c->par[2 + 6 * q];
@@ -889,7 +931,10 @@ static void ic_eval(const icdef_t *c, double x, double y, double U[8]) {
   U[1] = rho * v1;
   U[2] = rho * v2;
   U[3] = rho * v3;
-  U[4] = p / GAMMA1 + 0.5 * rho * (v1 * v1 + v2 * v2) + 0.5 * (B1 * B1 + B2 * B2);
+  if (c->full_energy)
+    U[4] = p / GAMMA1 + 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3) + 0.5 * (B1 * B1 + B2 * B2 + B3 * B3);
+  else
+    U[4] = p / GAMMA1 + 0.5 * rho * (v1 * v1 + v2 * v2) + 0.5 * (B1 * B1 + B2 * B2);
   U[5] = B1;
   U[6] = B2;
   U[7] = B3;
@@ -899,6 +944,8 @@ void orc_init_data(orc_state *s, int ic, const double *extra, int nextra) {
   icdef_t c;
   memset(&c, 0, sizeof c);
   c.ic = ic;
+  c.full_energy = (ic >= 5 && ic <= 9) || (ic >= 61 && ic <= 91);
+  c.shock = ic == ORC_IC_BLAST || ic == ORC_IC_CLOUD_SHOCK || ic == ORC_IC_CLOUD_SHOCK_2 || ic == ORC_IC_BRIO_WU_Y;
   if (ic == ORC_IC_OT_PERTURBED) {
     /* 16 modes, (kx, ky) = (1 + m mod 4, 1 + m div 4); phases from splitmix64 */
     c.par[0] = (nextra > 0) ? extra[0] : 1e-2;
@@ -965,6 +1012,78 @@ void orc_init_data(orc_state *s, int ic, const double *extra, int nextra) {
         s->By[BYI(i, j, d)] = acc / T->mmE[d - 1];
       }
     }
+  if (!c.shock) return;
+  /* init_data.f90:92-121: P0 states (cell and face-centre values) */
+  for (int j = 1; j <= Ny; j++)
+    for (int i = 1; i <= Nx; i++) {
+      double U[8];
+      ic_eval(&c, xa + (i - 0.5) * hx, ya + (j - 0.5) * hy, U);
+      for (int n = 1; n <= NE; n++) {
+        for (int d = 2; d <= NP; d++) s->uh[UHI(i, j, d, n)] = 0.0;
+        s->uh[UHI(i, j, 1, n)] = U[n - 1];
+      }
+    }
+  for (int j = 1; j <= Ny; j++)
+    for (int i = 0; i <= Nx; i++) {
+      double U[8];
+      ic_eval(&c, xa + i * hx, ya + (j - 0.5) * hy, U);
+      s->Bx[BXI(i, j, 1)] = U[5];
+      s->Bx[BXI(i, j, 2)] = s->Bx[BXI(i, j, 3)] = 0.0;
+    }
+  for (int j = 0; j <= Ny; j++)
+    for (int i = 1; i <= Nx; i++) {
+      double U[8];
+      ic_eval(&c, xa + (i - 0.5) * hx, ya + j * hy, U);
+      s->By[BYI(i, j, 1)] = U[6];
+      s->By[BYI(i, j, 2)] = s->By[BYI(i, j, 3)] = 0.0;
+    }
+}
+
+/* Loop_functions.f90:37-52 */
+static double loop_Az(double x, double y) {
+  double r = pow(x * x + y * y, 0.5);
+  return r < 0.3 ? 1e-3 * (0.3 - r) : 0.0;
+}
+
+void orc_calculate_Az(orc_state *s) {
+  int Nx = NXr, Ny = NYr;
+  double xa = s->p.xa, ya = s->p.ya, hx = s->hx, hy = s->hy;
+  for (int i = 0; i <= Nx; i++)
+    for (int j = 1; j <= Ny; j++) {
+      double X = xa + i * hx;
+      s->Bx[BXI(i, j, 1)] = -(1.0 / hy) * (loop_Az(X, ya + j * hy) - loop_Az(X, ya + (j - 1) * hy));
+    }
+  for (int i = 1; i <= Nx; i++)
+    for (int j = 0; j <= Ny; j++) {
+      double Y = ya + j * hy;
+      s->By[BYI(i, j, 1)] = (1.0 / hx) * (loop_Az(xa + i * hx, Y) - loop_Az(xa + (i - 1) * hx, Y));
+    }
+}
+
+void orc_calculate_L2error(orc_state *s, double t20, double L2[8]) {
+  int Nx = NXr, Ny = NYr;
+  const tables_t *T = &s->T;
+  icdef_t c;
+  memset(&c, 0, sizeof c);
+  c.ic = ORC_IC_VORTEX;
+  double xa = s->p.xa, ya = s->p.ya, hx = s->hx, hy = s->hy, hx1 = s->hx1, hy1 = s->hy1;
+  for (int n = 0; n < 8; n++) L2[n] = 0.0;
+  for (int i = 1; i <= Nx; i++)
+    for (int j = 1; j <= Ny; j++) {
+      double Xc = xa + (i - 0.5) * hx, Yc = ya + (j - 0.5) * hy;
+      for (int i1 = 0; i1 < NG; i1++)
+        for (int j1 = 0; j1 < NG; j1++) {
+          double U[8];
+          ic_eval(&c, Xc + hx1 * T->lambda[i1] - t20, Yc + hy1 * T->lambda[j1] - t20, U);
+          for (int n = 1; n <= NE; n++) {
+            double u = 0.0;
+            for (int d = 1; d <= NP; d++) u = u + s->uh[UHI(i, j, d, n)] * T->phiG[d - 1][i1][j1];
+            double e = u - U[n - 1];
+            L2[n - 1] = L2[n - 1] + T->weight[i1] * T->weight[j1] * (e * e);
+          }
+        }
+    }
+  for (int n = 0; n < 8; n++) L2[n] = pow(L2[n] * hx1 * hy1, 0.5);
 }
 
 /* ---------------------------------------------------------------------- */

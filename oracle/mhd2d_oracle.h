@@ -23,6 +23,8 @@ extern "C" {
 enum { ORC_LF = 1, ORC_HLL = 2, ORC_HLLC = 3, ORC_HLLD = 4 };
 enum { ORC_IND_NONE = 0, ORC_IND_ALL = 1, ORC_IND_KXRCF = 2 };
 enum { ORC_IC_ORSZAG_TANG = 3, ORC_IC_VORTEX = 1, ORC_IC_ROTOR = 4,
+       ORC_IC_BLAST = 5, ORC_IC_CLOUD_SHOCK = 6, ORC_IC_CLOUD_SHOCK_2 = 61, ORC_IC_MAGNETIC_LOOP = 7,
+       ORC_IC_BRIO_WU = 8, ORC_IC_BRIO_WU_Y = 81, ORC_IC_RYU_JONES = 9, ORC_IC_RYU_JONES_Y = 91,
        ORC_IC_OT_PERTURBED = 103, ORC_IC_RIEMANN4 = 104 };
 
 typedef struct orc_params {
@@ -58,6 +60,11 @@ void orc_set_time(orc_state *s, double t, double dt);
 
 /* IC projection (init_data.f90:21-121); extra[] = IC-family parameters */
 void orc_init_data(orc_state *s, int ic, const double *extra, int nextra);
+/* calculate_Az.f90:1-26: mode 1 of Bx, By from the magnetic loop's vector potential */
+void orc_calculate_Az(orc_state *s);
+/* calculate_L2error.f90:1-37: L2 error of each variable against the vortex
+   (init1.txt) translated by (t20, t20) */
+void orc_calculate_L2error(orc_state *s, double t20, double L2[8]);
 
 /* the reference's subroutines, one call each */
 void orc_set_bc(orc_state *s);

@@ -24,6 +24,8 @@ REF_DIR = os.path.join(HERE, "_ref")
 LF, HLL, HLLC, HLLD = 1, 2, 3, 4
 IND_NONE, IND_ALL, IND_KXRCF = 0, 1, 2
 IC_OT, IC_VORTEX, IC_ROTOR, IC_OT_PERTURBED, IC_RIEMANN4 = 3, 1, 4, 103, 104
+IC_BLAST, IC_CLOUD_SHOCK, IC_CLOUD_SHOCK_2, IC_MAGNETIC_LOOP = 5, 6, 61, 7
+IC_BRIO_WU, IC_BRIO_WU_Y, IC_RYU_JONES, IC_RYU_JONES_Y = 8, 81, 9, 91
 
 
 class OrcParams(C.Structure):
@@ -66,9 +68,11 @@ def lib():
         for name in ("orc_set_bc", "orc_Lh", "orc_LhEz", "orc_TVB_Limiter_P2", "orc_pp_Limiter",
                      "orc_KXRCF_Detector", "orc_div_free_Balsara", "orc_calculate_dt",
                      "orc_calculate_umax", "orc_calculate_pmin", "orc_calculate_totaldiv",
-                     "orc_rk3_prologue"):
+                     "orc_rk3_prologue", "orc_calculate_Az"):
             getattr(L, name).argtypes = [P]
             getattr(L, name).restype = None
+        L.orc_calculate_L2error.argtypes = [P, C.c_double, dp]
+        L.orc_calculate_L2error.restype = None
         L.orc_rk3_step.argtypes = [P, C.c_double]
         L.orc_rk3_step.restype = C.c_int
         L.orc_rk3_stages.argtypes = [P, C.c_double]
@@ -160,6 +164,11 @@ class Oracle:
             n = ex.size
         self.L.orc_init_data(self.h, ic, _dp(ex), n)
 
+    def calculate_L2error(self, t20):
+        out = np.zeros(8)
+        self.L.orc_calculate_L2error(self.h, t20, _dp(out))
+        return out
+
     def load(self, uh, Bx, By):
         self.uh[...] = uh
         self.Bx[...] = Bx
@@ -182,7 +191,7 @@ def ref_available(variant):
 
 
 def run_ref(variant, mode, uh, Bx, By, domain=(0.0, 2 * np.pi, 0.0, 2 * np.pi), bc=1, M=10.0,
-            tend=0.0, t=0.0, dt=0.0, nsteps=0, timeout=600):
+            tend=0.0, t=0.0, dt=0.0, nsteps=0, t20=0.0, timeout=600):
     """Run one mode of the reference driver; returns a dict of outputs."""
     exe = os.path.join(REF_DIR, variant)
     Nx, Ny = uh.shape[0] - 2, uh.shape[1] - 2
@@ -190,7 +199,7 @@ def run_ref(variant, mode, uh, Bx, By, domain=(0.0, 2 * np.pi, 0.0, 2 * np.pi), 
     hdr = np.zeros(16)
     hdr[0:4] = domain
     hdr[4:8] = bcs
-    hdr[8], hdr[9], hdr[10], hdr[11], hdr[12] = M, tend, t, dt, nsteps
+    hdr[8], hdr[9], hdr[10], hdr[11], hdr[12], hdr[13] = M, tend, t, dt, nsteps, t20
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
@@ -201,7 +210,8 @@ def run_ref(variant, mode, uh, Bx, By, domain=(0.0, 2 * np.pi, 0.0, 2 * np.pi), 
         subprocess.run([exe, mode, fin, fout], cwd=td, check=True, timeout=timeout,
                        stdout=subprocess.DEVNULL)
         raw = np.fromfile(fout, dtype=np.float64)
-    o = {}
+        l2 = np.fromfile(fout + ".l2", dtype=np.float64) if mode == "l2err" else None
+    o = {"L2": l2}
     k = 0
 
     def take(shape):

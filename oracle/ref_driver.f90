@@ -10,7 +10,9 @@
 !   INPUT : stream file = 16 doubles header + uh + Bx + By (reference layout)
 !           header: xa xb ya yb bcR bcL bcU bcD M tend t dt nsteps 0 0 0
 !   MODE  : setbc | lh | tvb | pp | divfree | kxrcf | dt | diag | run | bench
-!           | tables | eig | init3 | init1 | init4
+!           | tables | eig | init1 | init3 | init4 | init5 | init6 | init6b
+!           | init7 | init8 | init8b | init9 | init9b | az | l2err
+!           (l2err: t20 = header slot 14; the 8 errors go to OUTPUT.l2)
 !   OUTPUT: 8 scalars, 8 mesh values, uh, Bx, By, du, dEz1, dEz2, EzVertex,
 !           Is_Trouble_Cell (as doubles)
 program ref_driver
@@ -169,6 +171,36 @@ program ref_driver
         call init_data_1
     case ('init4')
         call init_data_4
+    case ('init5')
+        call init_data_5
+    case ('init6')
+        call init_data_6
+    case ('init6b')
+        call init_data_6b
+    case ('init7')
+        call init_data_7
+    case ('init8')
+        call init_data_8
+    case ('init8b')
+        call init_data_8b
+    case ('init9')
+        call init_data_9
+    case ('init9b')
+        call init_data_9b
+    case ('az')
+        call calculate_Az
+    case ('l2err')
+        do i = 1, Nx
+            Xc(i) = xa + (i - 0.5)*hx
+        end do
+        do j = 1, Ny
+            Yc(j) = ya + (j - 0.5)*hy
+        end do
+        t20 = hdr(14)
+        call calculate_L2error
+        open(12, file=trim(fout)//'.l2', access='stream', form='unformatted', status='replace')
+        write(12) L2
+        close(12)
     case default
         print *, 'unknown mode ', trim(mode)
         stop 2

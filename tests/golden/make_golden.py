@@ -200,5 +200,45 @@ def main():
          umax=r["umax"], totaldiv=r["totaldiv"], pmin=r["pmin"], trouble=r["trouble"])
 
 
+# initN.txt families beyond init1/3/4 (tag, driver mode, ic id); the run
+# fixtures use the defaults of the init file with the positivity limiter
+IC_LIBRARY = [("blast", "init5", 5), ("cloud", "init6", 6), ("cloud2", "init6b", 61), ("loop", "init7", 7),
+              ("briowu", "init8", 8), ("briowu_y", "init8b", 81), ("ryujones", "init9", 9),
+              ("ryujones_y", "init9b", 91)]
+
+
+def main_ic():
+    for tag, mode, ic in IC_LIBRARY:
+        r = O.run_ref("r16_f4", mode, *zeros(16, 16))
+        bc = 1 if ic == 7 else 2
+        save(f"init_{tag}_16", uh=pack(r["uh"]), Bx=r["Bx"], By=r["By"], ic=ic, bc=bc,
+             domain=np.array([r["xa"], r["xb"], r["ya"], r["yb"]]), M=r["M"], tend=r["tend"])
+    # calculate_Az.f90 on the magnetic loop
+    r0 = O.run_ref("r16_f4", "init7", *zeros(16, 16))
+    dom = np.array([r0["xa"], r0["xb"], r0["ya"], r0["yb"]])
+    r = O.run_ref("r16_f4", "az", r0["uh"], r0["Bx"], r0["By"], domain=dom)
+    save("az_loop_16", uh=pack(r0["uh"]), Bx0=r0["Bx"], By0=r0["By"], domain=dom, Bx=r["Bx"], By=r["By"])
+    # calculate_L2error.f90 on the projected vortex, translated by t20
+    r0 = O.run_ref("r16_f4", "init1", *zeros(16, 16))
+    dom = np.array([r0["xa"], r0["xb"], r0["ya"], r0["yb"]])
+    t20s = np.array([0.0, 0.3, 1.7])
+    L2 = np.stack([O.run_ref("r16_f4", "l2err", r0["uh"], r0["Bx"], r0["By"], domain=dom, t20=t)["L2"]
+                   for t in t20s])
+    save("l2err_vortex_16", uh=pack(r0["uh"]), Bx=r0["Bx"], By=r0["By"], domain=dom, t20=t20s, L2=L2)
+    # whole RK3 runs of shock problems (P0 data, outflow boundaries, positivity)
+    for tag, mode, tend in (("blast", "init5", 0.002), ("cloud", "init6", 0.02), ("briowu_y", "init8b", 0.02)):
+        r0 = O.run_ref("r16_f4_pp", mode, *zeros(16, 16))
+        dom = np.array([r0["xa"], r0["xb"], r0["ya"], r0["yb"]])
+        r = O.run_ref("r16_f4_pp", "run", r0["uh"], r0["Bx"], r0["By"], domain=dom, bc=2, M=r0["M"], tend=tend)
+        print(f"  run {tag}: {int(r['count'])} steps, t={r['t']}, pmin={r['pmin']:.3e}")
+        save(f"run_{tag}_f4_pp_16", uh0=pack(r0["uh"]), Bx0=r0["Bx"], By0=r0["By"], domain=dom, bc=2, M=r0["M"],
+             tend=tend, uh=pack(r["uh"]), Bx=r["Bx"], By=r["By"], t=r["t"], count=r["count"],
+             umax=r["umax"], totaldiv=r["totaldiv"], pmin=r["pmin"])
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["ic"]:
+        main_ic()
+    else:
+        main()
+        main_ic()

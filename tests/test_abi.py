@@ -41,3 +41,30 @@ def test_config_struct_layout(pkg):
     fields = re.findall(r"(int32_t|double)\s+([\w, ]+);", body)
     names = [n.strip() for _, group in fields for n in group.split(",")]
     assert names == [f[0] for f in pkg.Config._fields_]
+
+
+def test_info_struct_layout(pkg):
+    assert ctypes.sizeof(pkg.ICInfo) == 6 * 8 + 6 * 4
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct mhd2d_ic_info"):text.index("} mhd2d_ic_info;")]
+    fields = re.findall(r"(int32_t|double)\s+([\w, ]+);", body)
+    names = [n.strip() for _, group in fields for n in group.split(",")]
+    assert names == [f[0] for f in pkg.ICInfo._fields_]
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libmhd2d.so not built")
+@pytest.mark.parametrize("tag,ic", [("vortex", 1), ("ot", 3), ("rotor", 4), ("blast", 5), ("cloud", 6),
+                                    ("cloud2", 61), ("loop", 7), ("briowu", 8), ("briowu_y", 81),
+                                    ("ryujones", 9), ("ryujones_y", 91)])
+def test_ic_defaults_match_reference(pkg, tag, ic):
+    """mhd2d_ic_defaults (host only, no device) returns what the reference's
+    init_data leaves in its COMMON blocks for that initN.txt"""
+    from conftest import golden
+    g = golden(f"init_{tag}_16")
+    d = pkg.ic_defaults(ic)
+    assert d["domain"] == tuple(float(v) for v in g["domain"])
+    assert d["M"] == float(g["M"]) and d["tend"] == float(g["tend"])
+    if "bc" in g:
+        assert d["bc"] == (int(g["bc"]),) * 4
+    with pytest.raises(pkg.MHD2DError):
+        pkg.ic_defaults(12345)

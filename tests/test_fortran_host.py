@@ -24,20 +24,21 @@ N = 32
 STEPS = 3
 
 
-def run_host(tmp_path, mode, qtxt=False):
+def run_host(tmp_path, mode, qtxt=False, steps=STEPS, extra="", tag=""):
     if not os.path.exists(EXE):
         pytest.fail(f"{EXE} is not built (make -C {PKG_DIR} fortran)")
-    out = tmp_path / f"state_{mode}.bin"
-    nml = tmp_path / f"{mode}.nml"
+    out = tmp_path / f"state_{mode}{tag}.bin"
+    nml = tmp_path / f"{mode}{tag}.nml"
     nml.write_text(f"&run Nx_={N}, Ny_={N}, flux_type=4, ic=103, extra=1d-2, 20261203d0, nextra=2, "
-                   f"pp=1, M=50d0, CFL=0.2d0, tend_=1d9, max_steps={STEPS}, mode='{mode}', "
-                   f"out='{out}', qtxt={'.true.' if qtxt else '.false.'}, quiet=.true. /\n")
-    print(f"[fortran host] mode={mode} N={N} steps={STEPS}", flush=True)
+                   f"xa=0d0, xb=6.283185307179586d0, ya=0d0, yb=6.283185307179586d0, "
+                   f"pp=1, M=50d0, CFL=0.2d0, tend_=1d9, max_steps={steps}, mode='{mode}', "
+                   f"out='{out}', qtxt={'.true.' if qtxt else '.false.'}, quiet=.true. {extra}/\n")
+    print(f"[fortran host] mode={mode} N={N} steps={steps} {extra}", flush=True)
     r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = np.fromfile(out, dtype="<f8")
     t, count, nx, ny = raw[:4]
-    assert (int(nx), int(ny), int(count)) == (N, N, STEPS)
+    assert (int(nx), int(ny), int(count)) == (N, N, steps)
     n_uh, n_bx, n_by = (N + 2) * (N + 2) * 80, (N + 1) * N * 3, N * (N + 1) * 3
     uh = raw[4:4 + n_uh].reshape((N + 2, N + 2, 10, 8), order="F")
     Bx = raw[4 + n_uh:4 + n_uh + n_bx].reshape((N + 1, N, 3), order="F")
@@ -82,3 +83,14 @@ def test_host_output_files(tmp_path):
     assert np.allclose(w, np.polynomial.legendre.leggauss(5)[1], atol=1e-15)
     h = 2 * np.pi / N
     assert np.allclose(xc.reshape(N, 5).mean(axis=1), (np.arange(N) + 0.5) * h, atol=1e-14)
+
+
+def test_host_checkpoint_restart(tmp_path):
+    """save_checkpoint after 2 steps, restart in a new process to 5 steps:
+    bit-identical to 5 uninterrupted steps"""
+    ck = tmp_path / "ck.bin"
+    run_host(tmp_path, "fused", steps=2, extra=f"checkpoint='{ck}' ", tag="_a")
+    t1, u1, x1, y1, _ = run_host(tmp_path, "fused", steps=5, extra=f"restart='{ck}' ", tag="_b")
+    t0, u0, x0, y0, _ = run_host(tmp_path, "fused", steps=5, tag="_c")
+    assert t1 == t0
+    assert np.array_equal(u1, u0) and np.array_equal(x1, x0) and np.array_equal(y1, y0)

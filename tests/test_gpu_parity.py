@@ -108,7 +108,9 @@ RUNS = [("run_f4_16", 4, 0, 1), ("run_f2_16", 2, 0, 1), ("run_f1_16", 1, 0, 1), 
         ("run_f4_pp_16_outflow", 4, 1, 1), ("run_f4_pp_32", 4, 1, 1), ("run_f2_32", 2, 0, 1),
         ("run_rotor_f4_pp_32", 4, 1, 1),
         # TVB gated by KXRCF_Detector (Ck = 1)
-        ("run_f2_kx_32", 2, 0, 2), ("run_rotor_f4_pp_kx_32", 4, 1, 2)]
+        ("run_f2_kx_32", 2, 0, 2), ("run_rotor_f4_pp_kx_32", 4, 1, 2),
+        # shock problems from P0 data (init6, init8b), outflow boundaries
+        ("run_cloud_f4_pp_16", 4, 1, 1), ("run_briowu_y_f4_pp_16", 4, 1, 1)]
 
 
 # The rotor carries B = (B0, 0) in most of the domain: B is parallel to the x
@@ -118,8 +120,20 @@ RUNS = [("run_f4_16", 4, 0, 1), ("run_f2_16", 2, 0, 1), ("run_f1_16", 1, 0, 1), 
 # residual differs from the reference by ulps, which that square root turns
 # into ~1e-8 relative changes of the limiter's eigenvectors.  Its whole-run bar
 # is therefore 1e-9; on identical inputs every RK stage of the same run meets
-# 1e-12 (test_rotor_per_stage_identical_inputs).
-RUN_TOL = {"run_rotor_f4_pp_32": 1e-9, "run_rotor_f4_pp_kx_32": 1e-9}
+# 1e-12 (test_per_stage_identical_inputs).
+RUN_TOL = {"run_rotor_f4_pp_32": 1e-9, "run_rotor_f4_pp_kx_32": 1e-9,
+           # 87 steps; GPU vs oracle grows by ~1e-17 per step (measured), past 1e-12 at the end
+           "run_briowu_y_f4_pp_16": 5e-12}
+# The MHD blast (init5: B = 100/sqrt(4 pi), p = 0.1 outside the disk, plasma
+# beta ~ 1e-3) is not a whole-run case.  Its x-symmetric faces have a star
+# velocity S* that is zero up to rounding, and the reference's HLLC/HLLD flux
+# is not continuous across S* = 0 (its total pressure is gamma1 (E - rho
+# |u|^2 / 2), HLLD_Flux.f90:61-68): the star branch, hence an O(1) part of the
+# face flux, is picked by the sign of a rounding residue.  From the second
+# stage on (once the first limiter pass has broken exact mirror symmetry) the
+# GPU and the reference pick differently on a few faces; after a few steps
+# the reference's own state has negative pressure.  Its parity is checked on
+# the prologue and the first stage (test_per_stage_identical_inputs).
 
 
 @pytest.mark.parametrize("name", ["kxrcf_16", "kxrcf_rotor_32"])
@@ -192,13 +206,18 @@ def test_rk3_per_subroutine(pkg, oracle_mod, name, ft, pp):
             check(f"step {step} stage {stage} limited")
 
 
-def test_rotor_per_stage_identical_inputs(pkg, oracle_mod):
-    """every RK stage of the rotor run on identical inputs: the GPU gets the
-    oracle's state before each reference subroutine and must agree to 1e-12"""
+@pytest.mark.parametrize("name,steps", [("run_rotor_f4_pp_32", 3), ("run_blast_f4_pp_16", 1),
+                                        ("run_briowu_y_f4_pp_16", 30)])
+def test_per_stage_identical_inputs(pkg, oracle_mod, name, steps):
+    """every RK stage of a run on identical inputs: the GPU gets the oracle's
+    state before each reference subroutine and must agree to 1e-12.  For the
+    blast only the prologue and the first stage are compared (see the note
+    above RUN_TOL)."""
     O = oracle_mod
-    g = golden("run_rotor_f4_pp_32")
+    g = golden(name)
     s, uh0 = solver_from(pkg, g, uh_key="uh0", ft=4, pp=1)
-    o = O.Oracle(32, 32, flux_type=4, bc=1, domain=tuple(g["domain"]), M=float(g["M"]), pp=1)
+    N = uh0.shape[0] - 2
+    o = O.Oracle(N, N, flux_type=4, bc=int(g["bc"]), domain=tuple(g["domain"]), M=float(g["M"]), pp=1)
     o.load(uh0, g["Bx0"], g["By0"])
 
     def same_input():
@@ -211,12 +230,16 @@ def test_rotor_per_stage_identical_inputs(pkg, oracle_mod):
     o.set_bc()
     same_input(); s.TVB_Limiter_P2(); o.TVB_Limiter_P2(); cmp("prologue TVB")
     same_input(); s.div_free_Balsara(); o.div_free_Balsara(); cmp("prologue div_free")
-    for step in range(3):
+    compared = 0
+    for step in range(steps):
         o.calculate_dt()
         dt = o.scalars()["dt"]
         for stage in (1, 2, 3):
             if stage != 2:
                 o.set_bc()
+            if name.startswith("run_blast") and compared == 1:
+                return
+            compared += 1
             same_input(); s.Lh(); s.LhEz()
             o.Lh(); o.LhEz()
             assert rel_l2(s.debug_get(pkg.DU), o.du) < TOL, (step, stage, "Lh")

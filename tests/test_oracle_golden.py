@@ -130,8 +130,61 @@ def test_kxrcf_closed_form(oracle_mod):
     assert np.array_equal(o.trouble, expect)
 
 
+IC_LIBRARY = [("blast", 5), ("cloud", 6), ("cloud2", 61), ("loop", 7), ("briowu", 8), ("briowu_y", 81),
+              ("ryujones", 9), ("ryujones_y", 91)]
+
+
+@pytest.mark.parametrize("tag,ic", IC_LIBRARY)
+def test_init_library(oracle_mod, tag, ic):
+    """init_data.f90 with init5-9b.txt (incl. the P0 path of shock = 1)"""
+    O = oracle_mod
+    g = golden(f"init_{tag}_16")
+    o = O.Oracle(16, 16, domain=tuple(g["domain"]), bc=int(g["bc"]))
+    o.init_data(ic)
+    assert np.array_equal(interior(o.uh), interior(unpack(g["uh"])))
+    assert np.array_equal(o.Bx, g["Bx"]) and np.array_equal(o.By, g["By"])
+
+
+def test_init_library_shock_is_p0():
+    """shock = 1 ICs hold cell-centre values and no higher modes; the
+    smooth-projection ICs do not"""
+    for tag, ic in IC_LIBRARY:
+        uh = unpack(golden(f"init_{tag}_16")["uh"])
+        hi = np.abs(interior(uh)[:, :, 1:, :]).max()
+        if tag in ("blast", "cloud", "cloud2", "briowu_y"):
+            assert hi == 0.0, tag
+        else:
+            assert hi > 0.0, tag
+
+
+def test_calculate_az(oracle_mod):
+    O = oracle_mod
+    g = golden("az_loop_16")
+    o = O.Oracle(16, 16, domain=tuple(g["domain"]))
+    o.load(unpack(g["uh"]), g["Bx0"], g["By0"])
+    o.calculate_Az()
+    assert np.array_equal(o.Bx, g["Bx"]) and np.array_equal(o.By, g["By"])
+    # closed form: face averages of curl(Az) have zero discrete divergence
+    hx, hy = (g["domain"][1] - g["domain"][0]) / 16, (g["domain"][3] - g["domain"][2]) / 16
+    div = (o.Bx[1:, :, 0] - o.Bx[:-1, :, 0]) / hx + (o.By[:, 1:, 0] - o.By[:, :-1, 0]) / hy
+    assert np.abs(div).max() < 1e-15
+
+
+def test_calculate_l2error(oracle_mod):
+    O = oracle_mod
+    g = golden("l2err_vortex_16")
+    o = O.Oracle(16, 16, domain=tuple(g["domain"]))
+    o.load(unpack(g["uh"]), g["Bx"], g["By"])
+    for t20, ref in zip(g["t20"], g["L2"]):
+        assert np.array_equal(o.calculate_L2error(float(t20)), ref)
+    # the exact projection of a constant has zero error; w and Bz are zero
+    assert g["L2"][0][3] == 0.0 and g["L2"][0][7] == 0.0
+
+
 @pytest.mark.parametrize("name,ft,pp,ind", [("run_f4_16", 4, 0, 1), ("run_f2_16", 2, 0, 1), ("run_f1_16", 1, 0, 1),
                                             ("run_f3_16", 3, 0, 1), ("run_f4_pp_16_outflow", 4, 1, 1),
+                                            ("run_blast_f4_pp_16", 4, 1, 1), ("run_cloud_f4_pp_16", 4, 1, 1),
+                                            ("run_briowu_y_f4_pp_16", 4, 1, 1),
                                             ("run_f4_pp_32", 4, 1, 1), ("run_f2_32", 2, 0, 1),
                                             ("run_rotor_f4_pp_32", 4, 1, 1),
                                             ("run_f2_kx_32", 2, 0, 2), ("run_rotor_f4_pp_kx_32", 4, 1, 2)])
