@@ -252,9 +252,12 @@ def main():
               file=sys.stderr, flush=True)
     head = results[args.form]
     ms_per_step, value, prof, d = head["ms"], head["value"], head["prof"], head["diag"]
-    errors = None
+    errors = err_screen = None
     if args.form != "fp64" and args.error:
         errors = rel_l2_fields(head["state"][0], results["fp64"]["state"][0])
+    if args.form == "mixed_nn" and args.error and "mixed" in results:
+        # what the learned screen alone changes: same mixed arithmetic, KXRCF on every cell
+        err_screen = rel_l2_fields(head["state"][0], results["mixed"]["state"][0])
     dom_k = max(prof, key=lambda k: prof[k][0])
     ms_tot, launches = prof[dom_k]
     avg_ms = ms_tot / max(launches, 1)
@@ -283,6 +286,7 @@ def main():
                    "fp64_value": results["fp64"]["value"], "fp64_ms_per_step": results["fp64"]["ms"],
                    "speedup_vs_fp64": value / results["fp64"]["value"],
                    "rel_l2_vs_fp64": errors,
+                   "rel_l2_vs_mixed_kxrcf": err_screen,
                    "fp64_all_cells_value": results.get("fp64_all_cells", {}).get("value"),
                    "fp64_all_cells_ms_per_step": results.get("fp64_all_cells", {}).get("ms"),
                    "mixed_kxrcf_value": results.get("mixed", {}).get("value"),
