@@ -27,7 +27,9 @@ namespace {
 // in registers next to the 52 accumulators would spill, and re-reading them
 // from global memory misses L2 at this occupancy (measured: 2.5x the
 // algorithmic traffic).  In the plain form the epilogue takes S from LDS too.
-template <int MODE, typename T, int CBS, bool MIX = (sizeof(T) == 4)>
+// mixed form (T = float) on square cells (SQ): the volume fluxes are not
+// scaled by 1/hx1, 1/hy1 point by point; the volume sums are scaled once
+template <int MODE, typename T, int CBS, bool SQ = false, bool MIX = (sizeof(T) == 4)>
 __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ FX,
                                               const T* __restrict__ FY, SRd<MIX> A,
                                               SRd<MIX> U0, SWr<MIX> out, Geo g,
@@ -52,8 +54,11 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
 #pragma unroll
     for (int p = 0; p < NPL; p++) sm[p * CBS + tid] = T(S(p, c));
     const T ihx1 = T(1.0 / g.hx1), ihy1 = T(1.0 / g.hy1);
-#pragma unroll
-    for (int j1 = 0; j1 < 5; j1++) {
+    // one quadrature row; the mixed form runs rows and columns as
+    // compile-time loops (the coefficients and which of them vanish fold to
+    // constants), the plain form as a runtime row loop (measured faster in FP64)
+    auto row = [&](auto J1) {
+      const int j1 = J1;
       const double y = lam(j1);
       const T ty = T(y), tP2y = T(P2(y)), tP3y = T(P3(y));
       int off = tid;
@@ -78,8 +83,8 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
           c0[n] += m[PL(9, 6) * CBS] * tP3y;
         }
       }
-#pragma unroll
-      for (int i1 = 0; i1 < 5; i1++) {
+      auto col = [&](auto I1) {
+        const int i1 = I1;
         const double x = lam(i1);
         const T tx = T(x), tP2x = T(P2(x)), tP3x = T(P3(x));
         T Uq[8], Fx[8], Fy[8];
@@ -87,8 +92,10 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
         for (int n = 0; n < 8; n++) Uq[n] = c0[n] + c1[n] * tx + c2[n] * tP2x;
         Uq[5] += c3b * tP3x;
         lh_flux_xy<T>(Uq, Fx, Fy);
+        if (!SQ) {
 #pragma unroll
-        for (int n = 0; n < 8; n++) { Fx[n] *= ihx1; Fy[n] *= ihy1; }
+          for (int n = 0; n < 8; n++) { Fx[n] *= ihx1; Fy[n] *= ihy1; }
+        }
         const double ww = 0.25 * wgt(i1) * wgt(j1);
 #pragma unroll
         for (int n = 0; n < 8; n++) {
@@ -101,7 +108,23 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
             if (cy != 0.0) acc[p] += T(cy) * Fy[n];
           }
         }
+      };
+      if constexpr (MIX) {
+        static_for<0, 5>(col);
+      } else {
+#pragma unroll
+        for (int i1 = 0; i1 < 5; i1++) col(i1);
       }
+    };
+    if constexpr (MIX) {
+      static_for<0, 5>(row);
+    } else {
+#pragma unroll
+      for (int j1 = 0; j1 < 5; j1++) row(j1);
+    }
+    if (SQ) {
+#pragma unroll
+      for (int p = 0; p < NPL; p++) acc[p] *= ihx1;
     }
     // surface integrals
     const T sx = T(0.5 / g.hx), sy = T(0.5 / g.hy);
@@ -117,12 +140,28 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
         const T fu = FY[(q * 8 + n) * pl + c], fd = FY[(q * 8 + n) * pl + cD];
         acc0[n] -= sxd * wq * ((double)fr - (double)fl);
         acc0[n] -= syd * wq * ((double)fu - (double)fd);
+        if (sizeof(T) == 4) {
+          // phiTr(1, b) = +-phiTr(0, b) (the parity of mode b in xi), so each
+          // mode takes c (fr - fl) or c (fr + fl): one FMA per mode and face pair
+          const T sw = sx * T(wq), tw = sy * T(wq);
+          const T xd = (fr - fl) * sw, xs = (fr + fl) * sw, yd = (fu - fd) * tw, ys = (fu + fd) * tw;
 #pragma unroll
-        for (int b = 1; b < 10; b++) {
-          const int p = PL(b, n);
-          if (p < 0) continue;
-          acc[p] -= sx * T(wq) * (fr * T(phiTr(0, b, s, p2, p3, t23)) - fl * T(phiTr(1, b, s, p2, p3, t23)));
-          acc[p] -= sy * T(wq) * (fu * T(phiTr(2, b, s, p2, p3, t23)) - fd * T(phiTr(3, b, s, p2, p3, t23)));
+          for (int b = 1; b < 10; b++) {
+            const int p = PL(b, n);
+            if (p < 0) continue;
+            const double cr = phiTr(0, b, s, p2, p3, t23), cu = phiTr(2, b, s, p2, p3, t23);
+            const bool xodd = phiTr(1, b, s, p2, p3, t23) == -cr, yodd = phiTr(3, b, s, p2, p3, t23) == -cu;
+            if (cr != 0.0) acc[p] -= T(cr) * (xodd ? xs : xd);
+            if (cu != 0.0) acc[p] -= T(cu) * (yodd ? ys : yd);
+          }
+        } else {
+#pragma unroll
+          for (int b = 1; b < 10; b++) {
+            const int p = PL(b, n);
+            if (p < 0) continue;
+            acc[p] -= sx * T(wq) * (fr * T(phiTr(0, b, s, p2, p3, t23)) - fl * T(phiTr(1, b, s, p2, p3, t23)));
+            acc[p] -= sy * T(wq) * (fu * T(phiTr(2, b, s, p2, p3, t23)) - fd * T(phiTr(3, b, s, p2, p3, t23)));
+          }
         }
       }
     }
@@ -249,8 +288,12 @@ void launch_cell_t(mhd2d_ctx* ctx, const double* S, const double* A, const doubl
   const Geo& g = ctx->g;
   const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
   const long long pl = g.plane;
-  if (ctx->cfg.precision == MHD2D_MIXED)
-    hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
+  if (ctx->cfg.precision == MHD2D_MIXED && g.hx1 == g.hy1)
+    hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, true>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
+                       dim3(CBS_F32), 0, ctx->stream, SRd<true>{S, pl}, (const float*)ctx->FX, (const float*)ctx->FY,
+                       SRd<true>{A, pl}, SRd<true>{U0, pl}, SWr<true>{out, pl}, g, sc, dt);
+  else if (ctx->cfg.precision == MHD2D_MIXED)
+    hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, false>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
                        dim3(CBS_F32), 0, ctx->stream, SRd<true>{S, pl}, (const float*)ctx->FX, (const float*)ctx->FY,
                        SRd<true>{A, pl}, SRd<true>{U0, pl}, SWr<true>{out, pl}, g, sc, dt);
   else

@@ -45,12 +45,8 @@ __device__ __forceinline__ void trace_eval(const T cf[8][4], T s, T p2, T p3, T 
 // LhEz.f90:6-16 (face EMF).  Face i (0..Nx) of row j (1..Ny) lies between
 // cells i and i+1.
 template <int FT, typename T>
-__global__ __launch_bounds__(BS) void k_xface(SRd<true> U, T* __restrict__ FX,
-                                              T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
-  const long long f = xcd_block() * BS + threadIdx.x;
-  const long long nf = (long long)(g.Nx + 1) * g.Ny;
-  if (f >= nf) return;
-  const int i = (int)(f % (g.Nx + 1)), j = 1 + (int)(f / (g.Nx + 1));
+__device__ __forceinline__ void xface_at(const SRd<true>& U, T* __restrict__ FX, T* __restrict__ EZ,
+                                         const Geo& g, int i, int j) {
   const long long o = cix(g, i, j);
   T cL[8][4], cR[8][4];
   trace_coef<0, T>(U, o, cL);                 // right trace of cell i
@@ -77,16 +73,21 @@ __global__ __launch_bounds__(BS) void k_xface(SRd<true> U, T* __restrict__ FX,
     EZ[q * g.plane + o] = T(-0.5) * (FL[6] + FR[6] - S * (UR[6] - UL[6]));
   }
 }
+template <int FT, typename T>
+__global__ __launch_bounds__(BS) void k_xface(SRd<true> U, T* __restrict__ FX,
+                                              T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
+  const long long f = xcd_block() * BS + threadIdx.x;
+  const long long nf = (long long)(g.Nx + 1) * g.Ny;
+  if (f >= nf) return;
+  const int i = (int)(f % (g.Nx + 1)), j = 1 + (int)(f / (g.Nx + 1));
+  xface_at<FT, T>(U, FX, EZ, g, i, j);
+}
 
 // y faces: Lh.f90:139-203, :237-264; LhEz.f90:18-28.  Face j (0..Ny) of
 // column i (1..Nx) lies between cells j and j+1.
 template <int FT, typename T>
-__global__ __launch_bounds__(BS) void k_yface(SRd<true> U, T* __restrict__ FY,
-                                              T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
-  const long long f = xcd_block() * BS + threadIdx.x;
-  const long long nf = (long long)g.Nx * (g.Ny + 1);
-  if (f >= nf) return;
-  const int i = 1 + (int)(f % g.Nx), j = (int)(f / g.Nx);
+__device__ __forceinline__ void yface_at(const SRd<true>& U, T* __restrict__ FY, T* __restrict__ EZ,
+                                         const Geo& g, int i, int j) {
   const long long o = cix(g, i, j);
   T cD[8][4], cU[8][4];
   trace_coef<2, T>(U, o, cD);                 // upper trace of cell j (reference UU)
@@ -112,6 +113,15 @@ __global__ __launch_bounds__(BS) void k_yface(SRd<true> U, T* __restrict__ FY,
     const T S = tmax(fabs(UD[2] / UD[0]), fabs(UU[2] / UU[0]));
     EZ[q * g.plane + o] = T(0.5) * (FD[5] + FU[5] - S * (UU[5] - UD[5]));
   }
+}
+template <int FT, typename T>
+__global__ __launch_bounds__(BS) void k_yface(SRd<true> U, T* __restrict__ FY,
+                                              T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
+  const long long f = xcd_block() * BS + threadIdx.x;
+  const long long nf = (long long)g.Nx * (g.Ny + 1);
+  if (f >= nf) return;
+  const int i = 1 + (int)(f % g.Nx), j = (int)(f / g.Nx);
+  yface_at<FT, T>(U, FY, EZ, g, i, j);
 }
 
 // Plain form: one lane per (face, Gauss point); the trace at the point is
@@ -189,12 +199,7 @@ __device__ __forceinline__ void vertex_state(const SV& U, long long c, double ou
 }
 
 template <int FT, bool MIX>
-__global__ __launch_bounds__(BS) void k_vertex(SRd<MIX> U, double* __restrict__ EZV, Geo g,
-                                               const DevScal* __restrict__ sc) {
-  const long long t = xcd_block() * BS + threadIdx.x;
-  const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
-  if (t >= nv) return;
-  const int i = (int)(t % (g.Nx + 1)), j = (int)(t / (g.Nx + 1));
+__device__ __forceinline__ void vertex_at(const SRd<MIX>& U, double* __restrict__ EZV, const Geo& g, int i, int j) {
   double URU[8], ULU[8], URD[8], ULD[8];
   vertex_state<3>(U, cix(g, i + 1, j + 1), URU);  // LD corner of the upper-right cell
   vertex_state<2>(U, cix(g, i, j + 1), ULU);      // RD corner of the upper-left cell
@@ -202,6 +207,16 @@ __global__ __launch_bounds__(BS) void k_vertex(SRd<MIX> U, double* __restrict__ 
   vertex_state<0>(U, cix(g, i, j), ULD);          // RU corner of the lower-left cell
   EZV[cix(g, i, j)] = vertex_emf<double, FT>(URU, ULU, URD, ULD);
 }
+template <int FT, bool MIX>
+__global__ __launch_bounds__(BS) void k_vertex(SRd<MIX> U, double* __restrict__ EZV, Geo g,
+                                               const DevScal* __restrict__ sc) {
+  const long long t = xcd_block() * BS + threadIdx.x;
+  const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
+  if (t >= nv) return;
+  const int i = (int)(t % (g.Nx + 1)), j = (int)(t / (g.Nx + 1));
+  vertex_at<FT>(U, EZV, g, i, j);
+}
+
 
 }  // namespace
 
