@@ -127,34 +127,58 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
       for (int p = 0; p < NPL; p++) acc[p] *= ihx1;
     }
     // surface integrals
-    const T sx = T(0.5 / g.hx), sy = T(0.5 / g.hy);
-    const double sxd = 0.5 / g.hx, syd = 0.5 / g.hy;
     const long long cL = cix(g, i - 1, j), cD = cix(g, i, j - 1);
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-      const double s = lam(q), p2 = P2(s), p3 = P3(s), t23 = T23(s);
-      const double wq = wgt(q);
+    if constexpr (MIX) {
+      // face moments (FM_*): the trace of mode b on a face is trk(side, b, k)
+      // L_k(s) for one k, so its Gauss sum is trk * M_k.  trk(1, b, k) =
+      // +-trk(0, b, k) (the parity of b in xi), so each mode takes
+      // c (M_R - M_L) or c (M_R + M_L).
+      const T sx = T(0.5 / g.hx), sy = T(0.5 / g.hy);
+      const double sxd = 0.5 / g.hx, syd = 0.5 / g.hy;
 #pragma unroll
       for (int n = 0; n < 8; n++) {
-        const T fr = FX[(q * 8 + n) * pl + c], fl = FX[(q * 8 + n) * pl + cL];
-        const T fu = FY[(q * 8 + n) * pl + c], fd = FY[(q * 8 + n) * pl + cD];
-        acc0[n] -= sxd * wq * ((double)fr - (double)fl);
-        acc0[n] -= syd * wq * ((double)fu - (double)fd);
-        if (sizeof(T) == 4) {
-          // phiTr(1, b) = +-phiTr(0, b) (the parity of mode b in xi), so each
-          // mode takes c (fr - fl) or c (fr + fl): one FMA per mode and face pair
-          const T sw = sx * T(wq), tw = sy * T(wq);
-          const T xd = (fr - fl) * sw, xs = (fr + fl) * sw, yd = (fu - fd) * tw, ys = (fu + fd) * tw;
+        T mr[4], ml[4], mu[4], md[4];
 #pragma unroll
-          for (int b = 1; b < 10; b++) {
-            const int p = PL(b, n);
-            if (p < 0) continue;
-            const double cr = phiTr(0, b, s, p2, p3, t23), cu = phiTr(2, b, s, p2, p3, t23);
-            const bool xodd = phiTr(1, b, s, p2, p3, t23) == -cr, yodd = phiTr(3, b, s, p2, p3, t23) == -cu;
-            if (cr != 0.0) acc[p] -= T(cr) * (xodd ? xs : xd);
-            if (cu != 0.0) acc[p] -= T(cu) * (yodd ? ys : yd);
-          }
-        } else {
+        for (int k = 0; k < 3; k++) {
+          mr[k] = FX[(FM_N * n + k) * pl + c];
+          ml[k] = FX[(FM_N * n + k) * pl + cL];
+          mu[k] = FY[(FM_N * n + k) * pl + c];
+          md[k] = FY[(FM_N * n + k) * pl + cD];
+        }
+        mr[3] = ml[3] = mu[3] = md[3] = T(0);
+        if (n == 6) { mr[3] = FX[FM_P3 * pl + c]; ml[3] = FX[FM_P3 * pl + cL]; }
+        if (n == 5) { mu[3] = FY[FM_P3 * pl + c]; md[3] = FY[FM_P3 * pl + cD]; }
+        acc0[n] -= sxd * ((double)mr[0] - (double)ml[0]);
+        acc0[n] -= syd * ((double)mu[0] - (double)md[0]);
+        T xd[4], xs[4], yd[4], ys[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          xd[k] = (mr[k] - ml[k]) * sx; xs[k] = (mr[k] + ml[k]) * sx;
+          yd[k] = (mu[k] - md[k]) * sy; ys[k] = (mu[k] + md[k]) * sy;
+        }
+#pragma unroll
+        for (int b = 1; b < 10; b++) {
+          const int p = PL(b, n);
+          if (p < 0) continue;
+          const int kx = trace_k(0, b), ky = trace_k(2, b);
+          const double cr = trk(0, b, kx), cu = trk(2, b, ky);
+          acc[p] -= T(cr) * (trk(1, b, kx) == -cr ? xs[kx] : xd[kx]);
+          acc[p] -= T(cu) * (trk(3, b, ky) == -cu ? ys[ky] : yd[ky]);
+        }
+      }
+    } else {
+      const T sx = T(0.5 / g.hx), sy = T(0.5 / g.hy);
+      const double sxd = 0.5 / g.hx, syd = 0.5 / g.hy;
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        const double s = lam(q), p2 = P2(s), p3 = P3(s), t23 = T23(s);
+        const double wq = wgt(q);
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+          const T fr = FX[(q * 8 + n) * pl + c], fl = FX[(q * 8 + n) * pl + cL];
+          const T fu = FY[(q * 8 + n) * pl + c], fd = FY[(q * 8 + n) * pl + cD];
+          acc0[n] -= sxd * wq * ((double)fr - (double)fl);
+          acc0[n] -= syd * wq * ((double)fu - (double)fd);
 #pragma unroll
           for (int b = 1; b < 10; b++) {
             const int p = PL(b, n);
@@ -217,11 +241,16 @@ __global__ __launch_bounds__(BS) void k_ct(const T* __restrict__ EZX, const T* _
     base = 0;
     const double iy1 = 1.0 / g.hy1;
     double a[3] = {0.0, 0.0, 0.0};
+    if (sizeof(T) == 4) {   // mixed form: the face kernel stored sum_q w_q e_q and sum_q w_q s_q e_q
+      a[1] = 0.5 * iy1 * (double)EZX[o];
+      a[2] = iy1 * (double)EZX[pl + o];
+    } else {
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
-      const double e = 0.5 * wgt(q) * (double)EZX[q * pl + o];
-      a[1] += e * iy1;
-      a[2] += e * (2 * lam(q) * iy1);
+      for (int q = 0; q < 5; q++) {
+        const double e = 0.5 * wgt(q) * (double)EZX[q * pl + o];
+        a[1] += e * iy1;
+        a[2] += e * (2 * lam(q) * iy1);
+      }
     }
     const double vU = EZV[cix(g, i, j)], vD = EZV[cix(g, i, j - 1)];
 #pragma unroll
@@ -233,11 +262,16 @@ __global__ __launch_bounds__(BS) void k_ct(const T* __restrict__ EZX, const T* _
     base = 3;
     const double ix1 = 1.0 / g.hx1;
     double a[3] = {0.0, 0.0, 0.0};
+    if (sizeof(T) == 4) {
+      a[1] = -0.5 * ix1 * (double)EZY[o];
+      a[2] = -ix1 * (double)EZY[pl + o];
+    } else {
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
-      const double e = 0.5 * wgt(q) * (double)EZY[q * pl + o];
-      a[1] -= e * ix1;
-      a[2] -= e * (2 * lam(q) * ix1);
+      for (int q = 0; q < 5; q++) {
+        const double e = 0.5 * wgt(q) * (double)EZY[q * pl + o];
+        a[1] -= e * ix1;
+        a[2] -= e * (2 * lam(q) * ix1);
+      }
     }
     const double vR = EZV[cix(g, i, j)], vL = EZV[cix(g, i - 1, j)];
 #pragma unroll

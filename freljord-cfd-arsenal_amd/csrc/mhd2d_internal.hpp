@@ -107,6 +107,12 @@ constexpr int MLP_OFF_W3 = MLP_OFF_B2 + 64;          // [64]
 constexpr int MLP_OFF_B3 = MLP_OFF_W3 + 64;          // [1]
 constexpr int MLP_NPARAM = MLP_OFF_B3 + 1;           // 8129
 
+// mixed form: face moments M_k = sum_q w_q F(s_q) L_k(s_q) of the fluxes,
+// L = {1, s, P2(s)}, plane FM_N * n + k for variable n; plane FM_P3 holds
+// the P3(s) moment of the one variable that needs it (By on x faces, Bx on
+// y faces); EMF planes 0 and 1 hold its L_0 and L_1 moments
+constexpr int FM_N = 3, FM_P3 = 24;
+
 // kernel classes for mhd2d_profile_get
 enum { KC_XFACE = 0, KC_YFACE, KC_VERTEX, KC_CELL, KC_CT, KC_LIMIT, KC_SETBC, KC_DT, KC_KXRCF, KC_DIAG, KC_N };
 
@@ -214,7 +220,20 @@ int with_mix(const mhd2d_ctx* ctx, F&& f) {
 }
 
 // launch functions (stream-ordered on ctx->stream; 0 or a negative status)
+// calls f(integral_constant<int, flux_type>)
+template <typename F>
+int dispatch_ft(int ft, F&& f) {
+  switch (ft) {
+    case 1: return f(std::integral_constant<int, 1>{});
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 3: return f(std::integral_constant<int, 3>{});
+    default: return f(std::integral_constant<int, 4>{});
+  }
+}
+
 int launch_fluxes(mhd2d_ctx* ctx, const double* U, bool with_vertex, const DevScal* sc);     // mhd2d_faces.hip
+int launch_xface_mixed(mhd2d_ctx* ctx, const double* U, int ft);                            // mhd2d_faces_mix.hip
+int launch_yface_mixed(mhd2d_ctx* ctx, const double* U, int ft);                            // mhd2d_faces_mix.hip
 int launch_vertex(mhd2d_ctx* ctx, const double* U, const DevScal* sc);                      // mhd2d_faces.hip
 int launch_cell(mhd2d_ctx* ctx, int mode, const double* S, const double* A, const double* U0, double* out,
                 const DevScal* sc, double dt);                                                // mhd2d_cell.hip

@@ -175,6 +175,11 @@ __host__ __device__ __forceinline__ long long cix(const Geo& g, int i, int j) {
   return (long long)j * g.P + OFF + i;
 }
 
+// the one k with trk(side, b, k) != 0
+__host__ __device__ constexpr int trace_k(int side, int b) {
+  return trk(side, b, 0) != 0.0 ? 0 : trk(side, b, 1) != 0.0 ? 1 : trk(side, b, 2) != 0.0 ? 2 : 3;
+}
+
 // view of a state buffer (MIX: the mixed-form storage above); p is the plane
 // of PL(), c the cell index of cix()
 template <bool MIX>
