@@ -40,15 +40,18 @@ def alg_bytes(form):
     # one state: 52 FP64 planes; mixed storage: 22 FP64 planes (cell
     # averages, magnetic modes) + 30 FP32 planes (higher modes of the rest)
     S = 416 if form == "fp64" else 22 * 8 + 30 * 4
+    # values per face: fluxes and EMFs at the 5 Gauss points (plain form) or
+    # their Legendre moments (mixed form: 25 flux and 2 EMF moments)
+    NF, NE = (40, 5) if form == "fp64" else (25, 2)
     return {
-        "xface": S + 45 * B,          # state in; 40 fluxes + 5 face EMFs out
-        "yface": S + 45 * B,
+        "xface": S + (NF + NE) * B,   # state in; face fluxes and EMFs out
+        "yface": S + (NF + NE) * B,
         "vertex": S + 8,              # state in; one FP64 vertex EMF out
         # state + x/y fluxes in, stage value out; the RK inputs average
         # (0 + 2 + 2) / 3 state reads (stages 2 and 3 read U0 and the
         # unlimited previous stage value)
-        "cell": S + 2 * 40 * B + S + 4 * S / 3.0,
-        "ct": 2 * 5 * B + 8 + 48 + 48 + 4 * 48 / 3.0,   # face + vertex EMFs, faces in/out, RK inputs
+        "cell": S + 2 * NF * B + S + 4 * S / 3.0,
+        "ct": 2 * NE * B + 8 + 48 + 48 + 4 * 48 / 3.0,   # face + vertex EMFs, faces in/out, RK inputs
         # limiter pass: the detector/TVB kernel reads the state (KXRCF
         # traces), the apply pass reads state + faces and writes the state
         "limit": S + S + 48 + S + 8,
