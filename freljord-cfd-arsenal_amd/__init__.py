@@ -46,7 +46,7 @@ EXPORTS = (
 MLP_NPARAM = 8129        # W1[60][64], b1[64], W2[64][64], b2[64], w3[64], b3
 MLP_NIN = 60
 WEIGHTS_PATH = os.path.join(HERE, "indicator_weights.npz")
-STATS_KEYS = ("cells", "label", "kxrcf", "decision", "candidates", "missed", "over", "over_changed")
+STATS_KEYS = ("cells", "label", "kxrcf", "decision", "candidates", "missed", "over", "missed_hydro")
 KERNEL_CLASSES = ("xface", "yface", "vertex", "cell", "ct", "limit", "setbc", "dt", "kxrcf", "diag")
 
 

@@ -450,7 +450,8 @@ __global__ __launch_bounds__(BS) void k_istats(const int* __restrict__ screen, c
   const bool cand = candidate(screen, g, (int)(t % g.Nx), (int)(t / g.Nx));
   const bool k = kx[t] != 0, c = tm[t] != 0, lab = k && c;
   const bool d = cand && k;
-  const bool v[8] = {true, lab, k, d, cand, lab && !cand, cand && !lab, d && c && !k};
+  const bool hydro = (tm[t] & ~((1 << 5) | (1 << 6))) != 0;   // change bits of rho .. E, Bz
+  const bool v[8] = {true, lab, k, d, cand, lab && !cand, cand && !lab, lab && !cand && hydro};
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const unsigned long long b = __ballot(v[q]);

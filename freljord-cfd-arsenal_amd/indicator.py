@@ -25,6 +25,7 @@
    (the one-cell dilation of the screen), and it is the largest value that
    keeps the missed labels within the budget; the held-out statistics
    (mhd2d_indicator_stats) give the missed-shock and over-flag rates.
+   --evaluate <weights.npz> repeats step 4's statistics for saved weights.
 Resolutions: rotors and Riemann problems at --n; perturbed Orszag-Tang
 states also at 2x, 4x and 8x that (the benchmark runs the flow at 2048^2),
 with the negative sampling rate scaled as 256 / N.
@@ -189,7 +190,7 @@ def choose_lo(pm, y, miss_budget=1e-3):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", help="where the trained weights go (required unless --evaluate)")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--ics", type=int, default=64)
     ap.add_argument("--snap", type=int, default=6)
@@ -200,7 +201,10 @@ def main():
     ap.add_argument("--pos-weight", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--holdout", type=int, default=6)
+    ap.add_argument("--evaluate", metavar="WEIGHTS", help="only re-evaluate saved weights on the held-out set")
     args = ap.parse_args()
+    if not args.out and not args.evaluate:
+        ap.error("--out is required")
     P = _pkg()
     rng = np.random.default_rng(args.seed)
     kinds = ["rotor", "riemann", "ot"]
@@ -210,6 +214,15 @@ def main():
         ics += [(N * m, random_ic(rng, "ot", N * m)) for _ in range(cnt)]
     held = [(N, random_ic(rng, kinds[k % 3], N)) for k in range(args.holdout)]
     held += [(N * 8, random_ic(rng, "ot", N * 8))]
+    if args.evaluate:
+        with np.load(args.evaluate, allow_pickle=False) as z:
+            params, lo = np.asarray(z["params"], dtype=np.float32), float(z["mlp_lo"])
+        stats, _, _ = evaluate(params, held, 3, lo)
+        rep = {"mlp_lo": lo, "heldout": stats, "missed_rate": stats["missed"] / max(stats["label"], 1),
+               "missed_hydro_rate": stats["missed_hydro"] / max(stats["label"], 1),
+               "candidate_fraction": stats["candidates"] / max(stats["cells"], 1)}
+        print("[indicator] " + json.dumps(rep), flush=True)
+        return
     tr = P.Trainer(capacity=4_000_000, max_batch=args.batch)
     n, npos = build_dataset(tr, ics, args.snap, args.neg_keep, args.seed)
     print(f"[indicator] dataset: {n} rows, {npos} positive", flush=True)
@@ -231,6 +244,7 @@ def main():
     rep = {"rows": n, "positives": npos, "train_seconds": train_s, "losses": losses, "mlp_lo": lo, "mlp_hi": hi,
            "heldout": stats,
            "missed_rate": stats["missed"] / max(stats["label"], 1),
+           "missed_hydro_rate": stats["missed_hydro"] / max(stats["label"], 1),
            "over_flag_rate": stats["over"] / max(stats["candidates"], 1),
            "candidate_fraction": stats["candidates"] / max(stats["cells"], 1)}
     print("[indicator] " + json.dumps(rep), flush=True)

@@ -169,8 +169,11 @@ int mhd2d_mlp_probability(mhd2d_ctx *ctx, float *host_p);
 /* indicator statistics on the current state, out[8] = { cells, label
    (KXRCF flags AND TVB changes), KXRCF flags, decisions (candidates KXRCF
    flags: TVB applied), candidates (cells handed to KXRCF), missed (label and
-   not a candidate), over (candidate and not label), TVB changes the plain
-   method would not make (0 by construction) } */
+   not a candidate), over (candidate and not label), missed_hydro (missed
+   cells whose TVB change includes a non-magnetic variable: changes to Bx, By
+   alone are overwritten by div_free_Balsara) }.  The screen can only skip
+   TVB applications, never add one: the decisions are a subset of the plain
+   method's. */
 int mhd2d_indicator_stats(mhd2d_ctx *ctx, int64_t *out);
 
 /* FP32 training of the indicator on the GPU.  A trainer owns a device
