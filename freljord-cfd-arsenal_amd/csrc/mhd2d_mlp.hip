@@ -157,6 +157,14 @@ struct MlpDev {
   float b1[64], b2[64], w3[64], b3;
 };
 
+// the screen's LDS tiles are per wave (sX[wv], sZ[wv]): a wave orders its
+// own LDS writes before its reads, without waiting for the block's other waves
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool MIX>
 __global__ __launch_bounds__(256) void k_mlp_screen(SRd<MIX> U, int* __restrict__ screen,
                                                     float* __restrict__ prob, const MlpDev* __restrict__ W,
@@ -178,7 +186,7 @@ __global__ __launch_bounds__(256) void k_mlp_screen(SRd<MIX> U, int* __restrict_
 #pragma unroll
     for (int k = MLP_NIN / 2; k < 32; k++) X32[k] = 0u;
   }
-  __syncthreads();
+  wave_lds_sync();
   // layer 1: [64 cells x 64] x [64 x 64]
   f4 acc[4][4];
 #pragma unroll
@@ -197,7 +205,7 @@ __global__ __launch_bounds__(256) void k_mlp_screen(SRd<MIX> U, int* __restrict_
       for (int nt = 0; nt < 4; nt++) acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[nt], acc[mt][nt], 0, 0, 0);
     }
   }
-  __syncthreads();
+  wave_lds_sync();
 #pragma unroll
   for (int mt = 0; mt < 4; mt++)
 #pragma unroll
@@ -205,7 +213,7 @@ __global__ __launch_bounds__(256) void k_mlp_screen(SRd<MIX> U, int* __restrict_
 #pragma unroll
       for (int q = 0; q < 4; q++)
         X[(mt * 16 + 4 * grp + q) * XP + nt * 16 + col] = bf16_bits(gelu_fast(acc[mt][nt][q] + W->b1[nt * 16 + col]));
-  __syncthreads();
+  wave_lds_sync();
   // layer 2
 #pragma unroll
   for (int mt = 0; mt < 4; mt++)
@@ -238,7 +246,7 @@ __global__ __launch_bounds__(256) void k_mlp_screen(SRd<MIX> U, int* __restrict_
       s += __shfl_xor(s, 8);
       if (col == 0) sZ[wv][mt * 16 + 4 * grp + q] = s + W->b3;
     }
-  __syncthreads();
+  wave_lds_sync();
   if (!valid) return;
   const float p = __builtin_amdgcn_rcpf(1.0f + __expf(-sZ[wv][l]));
   prob[t] = p;
