@@ -86,6 +86,11 @@ def measured_traffic(kernel, form, N):
     return None
 
 
+def indicator_mod():
+    import importlib
+    return importlib.import_module("freljord_cfd_arsenal_amd.indicator")
+
+
 def import_pkg():
     name = "freljord_cfd_arsenal_amd"
     spec = importlib.util.spec_from_file_location(name, os.path.join(PKG_DIR, "__init__.py"),
@@ -224,7 +229,6 @@ def main():
         state = s.download() if args.error else None
         diag = s.diagnostics()
         tnow = s.get_time()[0]
-        istats = s.indicator_stats() if (nn and ind == pkg.IND_MLP) else None
         prof = None
         if profile_steps:
             s.profile(True)
@@ -232,6 +236,8 @@ def main():
                 s.step()
             prof = s.profile_read()
             s.profile(False)
+        # screen vs plain method on the next stage-1 limiter input (changes the state: last)
+        istats = indicator_mod().limiter_input_stats(s) if (nn and ind == pkg.IND_MLP) else None
         s.close()
         return dict(secs=secs, state=state, diag=diag, t=tnow, prof=prof, ms=1e3 * secs / steps,
                     value=DOF_PER_CELL * N * N * steps / secs, istats=istats)

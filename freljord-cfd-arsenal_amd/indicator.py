@@ -120,6 +120,18 @@ def snapshots(ic, N, nsnap, on_state, cfl=0.2, band=(0.2, 0.8)):
     probe.close()
 
 
+def limiter_input_stats(solver):
+    """indicator statistics on a limiter input: advances `solver` by one
+    unlimited stage (set_bc, Lh, LhEz, dt, UI = S + dt L(S)), the state the
+    stage-1 limiter sees, and evaluates the screen and the plain method
+    there.  The solver's state is left at that stage value."""
+    solver.set_bc()
+    solver.Lh()
+    solver.LhEz()
+    solver.rk_stage(1, solver.calculate_dt())
+    return solver.indicator_stats()
+
+
 def build_dataset(trainer, ics, nsnap, neg_keep, seed, log=print):
     """ics: list of (N, ic); the negative sampling rate is neg_keep at 256^2
     and scales as 256 / N, so the finer grids add more rows per snapshot
