@@ -166,7 +166,9 @@ int mhd2d_mlp_set_weights(mhd2d_ctx *ctx, const float *params, int32_t n);
 /* indicator probability of every interior cell of the current state;
    host_p is (Nx, Ny) column-major (may be NULL) */
 int mhd2d_mlp_probability(mhd2d_ctx *ctx, float *host_p);
-/* indicator statistics on the current state, out[8] = { cells, label
+/* indicator statistics on the current state, read as a limiter input (its
+   halos as they are: after mhd2d_rk_stage 1 that is the stage-1 limiter's
+   input), out[8] = { cells, label
    (KXRCF flags AND TVB changes), KXRCF flags, decisions (candidates KXRCF
    flags: TVB applied), candidates (cells handed to KXRCF), missed (label and
    not a candidate), over (candidate and not label), missed_hydro (missed
