@@ -94,3 +94,26 @@ def test_host_checkpoint_restart(tmp_path):
     t0, u0, x0, y0, _ = run_host(tmp_path, "fused", steps=5, tag="_c")
     assert t1 == t0
     assert np.array_equal(u1, u0) and np.array_equal(x1, x0) and np.array_equal(y1, y0)
+
+
+def test_host_l2error_vortex(tmp_path, pkg):
+    """calculate_L2error through the Fortran host (main.f90's t20 = tend)
+    against the C ABI on the same run: the smooth vortex (init1.txt, domain
+    and boundaries from mhd2d_ic_defaults) advanced to t = 0.05 at 16^2"""
+    nml = tmp_path / "l2.nml"
+    nml.write_text("&run Nx_=16, Ny_=16, flux_type=4, ic=1, pp=0, indicator=1, CFL=0.15d0, tend_=0.05d0, "
+                   "l2err=.true. /\n")
+    r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = r.stdout.splitlines()
+    k = max(i for i, l in enumerate(out) if l.lstrip().startswith("t = "))
+    L2_host = np.array([float(v) for l in out[k + 1:] for v in l.split()])
+    d = pkg.ic_defaults(pkg.IC_VORTEX)
+    s = pkg.Solver(16, 16, flux_type=pkg.HLLD, bc=d["bc"], domain=d["domain"], M=d["M"], CFL=0.15)
+    s.init_data(pkg.IC_VORTEX)
+    t, _ = s.RK3(0.05)
+    L2 = s.calculate_L2error(t)
+    print(f"[fortran host] L2 error at t = {t}: {L2_host}")
+    assert L2_host.shape == (8,)
+    assert np.allclose(L2_host, L2, rtol=1e-12, atol=1e-300)
+    assert L2[0] > 0 and L2[3] == 0.0 and L2[7] == 0.0

@@ -372,3 +372,29 @@ def test_mixed_storage_roundtrip(pkg):
                 assert np.array_equal(a, r), (b, n)
             else:
                 assert np.array_equal(a, r.astype(np.float32).astype(np.float64)), (b, n)
+
+
+def test_mixed_shock_rotor(pkg):
+    """the mixed form on a shock flow (the reference's rotor, init4.txt, with
+    positivity and KXRCF-gated TVB): stays finite and divergence-free and
+    close to the plain form (measured 6e-6 in density after 20 steps at
+    128^2).  FP32 rounding can move a limiter decision at a discontinuity, so
+    the bar is looser than the smooth benchmark flow's 1e-5
+    (test_mixed_vs_fp64)."""
+    N = 128
+    out = {}
+    for prec in (pkg.FP64, pkg.MIXED):
+        s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=(0.0, 1.0, 0.0, 1.0), M=10.0, CFL=0.15, pp=1,
+                       indicator=pkg.IND_KXRCF, precision=prec)
+        s.init_data(pkg.IC_ROTOR)
+        s.rk3_prologue()
+        for _ in range(20):
+            s.step()
+        out[prec] = (s.download()[0], s.diagnostics())
+    (u64, d64), (u32, d32) = out[pkg.FP64], out[pkg.MIXED]
+    err = rel_l2(interior(u32)[:, :, :, 0], interior(u64)[:, :, :, 0])
+    print(f"[mixed] rotor {N}^2, 20 steps: density rel L2 mixed vs FP64 {err:.3e}, "
+          f"totaldiv {d32['totaldiv']:.2e}, pmin {d32['pmin']:.3e} / {d64['pmin']:.3e}")
+    assert np.isfinite(u32).all()
+    assert d32["totaldiv"] < 1e-12
+    assert err < 1e-4
