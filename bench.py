@@ -218,12 +218,11 @@ def main():
             s.mlp_set_weights(nn[0])
         s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
         s.rk3_prologue()
-        for _ in range(warmup):
-            s.step()
+        s.step_graph(0)            # capture the step's HIP graphs (six buffer-role phases) ahead
+        s.step_graph(warmup)
         s.sync()
         t0 = time.perf_counter()
-        for _ in range(steps):
-            s.step()
+        s.step_graph(steps)        # one graph launch per step, no host round trip
         s.sync()
         secs = time.perf_counter() - t0
         state = s.download() if args.error else None

@@ -34,7 +34,7 @@ EXPORTS = (
     "mhd2d_download", "mhd2d_init_data", "mhd2d_set_bc", "mhd2d_lh", "mhd2d_lhez",
     "mhd2d_rk_stage", "mhd2d_tvb_limiter_p2", "mhd2d_pp_limiter", "mhd2d_kxrcf_detector",
     "mhd2d_div_free_balsara", "mhd2d_calculate_dt", "mhd2d_diagnostics", "mhd2d_rk3_prologue",
-    "mhd2d_step", "mhd2d_get_time", "mhd2d_set_time", "mhd2d_debug_get",
+    "mhd2d_step", "mhd2d_step_graph", "mhd2d_get_time", "mhd2d_set_time", "mhd2d_debug_get",
     "mhd2d_profile_enable", "mhd2d_profile_get",
     "mhd2d_mlp_set_weights", "mhd2d_mlp_probability", "mhd2d_indicator_stats",
     "mhd2d_trainer_create", "mhd2d_trainer_destroy", "mhd2d_trainer_last_error", "mhd2d_trainer_collect",
@@ -109,6 +109,7 @@ def load_library(path: str = LIB_PATH):
         "mhd2d_diagnostics": ([P, dp, dp, dp], C.c_int),
         "mhd2d_rk3_prologue": ([P], C.c_int),
         "mhd2d_step": ([P, C.c_double, dp], C.c_int),
+        "mhd2d_step_graph": ([P, C.c_double, C.c_int64, dp], C.c_int),
         "mhd2d_get_time": ([P, dp, dp, i64p], C.c_int),
         "mhd2d_set_time": ([P, C.c_double, C.c_int64], C.c_int),
         "mhd2d_debug_get": ([P, i32, dp], C.c_int),
@@ -297,6 +298,12 @@ class Solver:
         self._chk(self.L.mhd2d_step(self.h, tend, C.byref(t) if sync else None), "step")
         return t.value if sync else None
 
+    def step_graph(self, nsteps, tend=float("inf"), sync=False):
+        """nsteps steps replayed as HIP graph launches (mhd2d_step_graph)"""
+        t = C.c_double()
+        self._chk(self.L.mhd2d_step_graph(self.h, tend, int(nsteps), C.byref(t) if sync else None), "step_graph")
+        return t.value if sync else None
+
     def get_time(self):
         t, dt, n = C.c_double(), C.c_double(), C.c_int64()
         self._chk(self.L.mhd2d_get_time(self.h, C.byref(t), C.byref(dt), C.byref(n)), "get_time")
@@ -334,14 +341,16 @@ class Solver:
         self._chk(self.L.mhd2d_indicator_stats(self.h, _i64p(out)), "indicator_stats")
         return dict(zip(STATS_KEYS, (int(v) for v in out)))
 
-    def RK3(self, tend, max_steps=None):
-        """RK3.f90:5-174 without frame output: prologue, then steps while t < tend."""
+    def RK3(self, tend, max_steps=None, chunk=16):
+        """RK3.f90:5-174 without frame output: prologue, then steps while
+        t < tend, as graph launches in chunks of `chunk` steps (the steps of a
+        chunk that start at t >= tend change nothing); returns (t, steps)."""
         self.rk3_prologue()
-        t = 0.0
-        n = 0
+        t, n = 0.0, 0
         while t < tend and (max_steps is None or n < max_steps):
-            t = self.step(tend, sync=True)
-            n += 1
+            k = chunk if max_steps is None else min(chunk, max_steps - n)
+            self.step_graph(k, tend)
+            t, _, n = self.get_time()
         return t, n
 
 

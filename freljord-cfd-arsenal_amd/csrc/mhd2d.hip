@@ -32,6 +32,7 @@ namespace {
 // x-then-y order of the reference makes the corners (map_x(i), map_y(j)).
 template <bool MIX>
 __global__ __launch_bounds__(BS) void k_setbc(SWr<MIX> U, Geo g, int nplanes, const DevScal* __restrict__ sc) {
+  if (step_off(g)) return;
   const long long t = (long long)blockIdx.x * BS + threadIdx.x;
   const int W = g.Nx + 2, H = g.Ny + 2;
   const long long ring = 2LL * W + 2LL * g.Ny;
@@ -121,19 +122,28 @@ __global__ __launch_bounds__(BS) void k_dt(SRd<MIX> U, Geo g, DevScal* sc) {
 // t >= tend are switched off (active = 0) like the reference's loop test.
 __global__ void k_dt_final(DevScal* sc, double hx, double hy, int advance) {
   if (threadIdx.x != 0) return;
+  if (advance && !(sc->t < sc->tend)) { sc->active = 0; return; }
   const double ax = __longlong_as_double((long long)sc->ax_bits);
   const double ay = __longlong_as_double((long long)sc->ay_bits);
   sc->alphax = ax;
   sc->alphay = ay;
   double dt = sc->CFL / (ax / hx + ay / hy);
   if (!advance) { sc->dt = dt; return; }
-  if (!(sc->t < sc->tend)) { sc->active = 0; return; }
   sc->active = 1;
   if (sc->t + dt >= sc->tend) { dt = sc->tend - sc->t; sc->t = sc->tend; }
   else sc->t = sc->t + dt;
   if (dt < 1e-7) sc->t = sc->tend;
   sc->dt = dt;
   sc->count += 1;
+}
+
+// a step that started at t >= tend ran none of its kernels: its state and
+// face B move to the buffers that hold them after the step (the host rotates
+// the buffer roles every step)
+__global__ __launch_bounds__(BS) void k_carry(const double* __restrict__ src, double* __restrict__ dst, long long n,
+                                              const int* gate) {
+  if (!gate || *gate != 0) return;
+  for (long long t = (long long)blockIdx.x * BS + threadIdx.x; t < n; t += (long long)gridDim.x * BS) dst[t] = src[t];
 }
 
 // calculate_umax.f90, calculate_pmin.f90 and per-block partial sums of
@@ -543,6 +553,7 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
   if (!ctx) return -1;
   new (&ctx->prof_ev) decltype(ctx->prof_ev)();
   new (&ctx->ev_pool) decltype(ctx->ev_pool)();
+  new (&ctx->graphs) decltype(ctx->graphs)();
   ctx->cfg = *cfg;
   if (ctx->cfg.CFL <= 0) ctx->cfg.CFL = 0.15;
   Geo& g = ctx->g;
@@ -609,6 +620,8 @@ int mhd2d_destroy(mhd2d_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& pe : ctx->prof_ev) { (void)hipEventDestroy(pe.second.first); (void)hipEventDestroy(pe.second.second); }
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  for (auto& sg : ctx->graphs) (void)hipGraphExecDestroy(sg.exec);
+  ctx->graphs.~vector();
   ctx->prof_ev.~vector();
   ctx->ev_pool.~vector();
   void* ps[] = {ctx->stage, ctx->S, ctx->U0, ctx->UI, ctx->UII, ctx->T, ctx->du, ctx->F, ctx->F0, ctx->FI, ctx->FII,
@@ -1176,11 +1189,19 @@ int mhd2d_rk3_prologue(mhd2d_ctx* ctx) {
   return launch_diag(ctx);
 }
 
-// one pass of the RK3.f90:135-228 loop body with t and dt on the device
-int mhd2d_step(mhd2d_ctx* ctx, double tend, double* t_out) {
-  int rc = ctx_check(ctx);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_set_tend, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, tend);
+}  // extern "C"
+
+namespace {
+
+// the kernels of one pass of the RK3.f90:135-228 loop body, from dt to the
+// new state (tend is set by the caller); rotates the buffer roles
+int step_body(mhd2d_ctx* ctx) {
+  struct GateOn {
+    mhd2d_ctx* c;
+    explicit GateOn(mhd2d_ctx* c_) : c(c_) { c->g.gate = &c->sc->active; }
+    ~GateOn() { c->g.gate = nullptr; }
+  } gate_on(ctx);
+  int rc;
   if ((rc = launch_dt(ctx, 1))) return rc;
   DevScal* sc = ctx->sc;
   const bool pp = ctx->cfg.pp != 0;
@@ -1207,15 +1228,121 @@ int mhd2d_step(mhd2d_ctx* ctx, double tend, double* t_out) {
   if ((rc = launch_cell(ctx, 3, ctx->S, ctx->UII, ctx->U0, ctx->T, sc, 0.0))) return rc;
   if ((rc = launch_ct(ctx, 3, nullptr, ctx->FII, ctx->F0, ctx->F, sc, 0.0))) return rc;
   if ((rc = launch_limit(ctx, ctx->T, ctx->UI, ctx->F, true, pp, true, sc, 3))) return rc;
+  // a switched-off step: the start-of-step state and faces become the new ones
+  const long long ns = (long long)(state_bytes(ctx->cfg.precision == MHD2D_MIXED, ctx->g.plane) / sizeof(double));
+  const long long nf = 6LL * ctx->g.plane;
+  const unsigned nbc = std::min<unsigned>(nblk(ns), 1024u);
+  hipLaunchKernelGGL(k_carry, dim3(nbc), dim3(BS), 0, ctx->stream, (const double*)ctx->U0, ctx->UI, ns, ctx->g.gate);
+  hipLaunchKernelGGL(k_carry, dim3(nbc), dim3(BS), 0, ctx->stream, (const double*)ctx->F0, ctx->F, nf, ctx->g.gate);
+  LAUNCHCHK(ctx);
   std::swap(ctx->S, ctx->UI);   // S <- new state (halos refreshed by the next step's set_bc); UI is free
   ctx->lh_done = false;
-  if (t_out) {
-    DevScal h;
-    HIPCHK(ctx, hipMemcpyAsync(&h, ctx->sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    *t_out = h.t;
-  }
   return 0;
+}
+
+// the buffers whose roles rotate from step to step (period 6)
+constexpr int NROT = 9;
+void rot_get(const mhd2d_ctx* c, double* p[NROT]) {
+  double* const v[NROT] = {c->S, c->U0, c->UI, c->UII, c->T, c->F, c->F0, c->FI, c->FII};
+  for (int k = 0; k < NROT; k++) p[k] = v[k];
+}
+void rot_set(mhd2d_ctx* c, double* const p[NROT]) {
+  c->S = p[0]; c->U0 = p[1]; c->UI = p[2]; c->UII = p[3]; c->T = p[4];
+  c->F = p[5]; c->F0 = p[6]; c->FI = p[7]; c->FII = p[8];
+}
+
+// captures step_body for the current buffer roles; the roles are left as
+// they were (nothing has run) and the ones after the step go to sg.after
+int capture_step(mhd2d_ctx* ctx, StepGraph& sg) {
+  rot_get(ctx, sg.before);
+  hipGraph_t graph = nullptr;
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = step_body(ctx);
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+  rot_get(ctx, sg.after);
+  rot_set(ctx, sg.before);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(ctx, -2, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&sg.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(ctx, -2, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+  return 0;
+}
+
+int read_t(mhd2d_ctx* ctx, double* t_out) {
+  if (!t_out) return 0;
+  DevScal h;
+  HIPCHK(ctx, hipMemcpyAsync(&h, ctx->sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *t_out = h.t;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// one pass of the RK3.f90:135-228 loop body with t and dt on the device
+int mhd2d_step(mhd2d_ctx* ctx, double tend, double* t_out) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_set_tend, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, tend);
+  if ((rc = step_body(ctx))) return rc;
+  return read_t(ctx, t_out);
+}
+
+// nsteps passes of the loop body, each one launch of a HIP graph captured
+// (once per buffer-role phase) from the same kernels as mhd2d_step
+int mhd2d_step_graph(mhd2d_ctx* ctx, double tend, int64_t nsteps, double* t_out) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(ctx, -1, "mhd2d_step_graph: nsteps = %lld < 0", (long long)nsteps);
+  // the graph of the current buffer roles (captured when missing)
+  auto graph_now = [&](StepGraph*& sg) -> int {
+    double* cur[NROT];
+    rot_get(ctx, cur);
+    sg = nullptr;
+    for (auto& e : ctx->graphs)
+      if (std::equal(cur, cur + NROT, e.before)) { sg = &e; return 0; }
+    StepGraph n{};
+    if (int r = capture_step(ctx, n)) return r;
+    if (ctx->graphs.size() >= 8) {   // only six phases occur while stepping
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      (void)hipGraphExecDestroy(ctx->graphs.front().exec);
+      ctx->graphs.erase(ctx->graphs.begin());
+    }
+    ctx->graphs.push_back(n);
+    sg = &ctx->graphs.back();
+    return 0;
+  };
+  StepGraph* sg = nullptr;
+  if (nsteps == 0) {   // capture the six phases ahead, run nothing
+    if (ctx->prof) return 0;
+    double* start[NROT];
+    rot_get(ctx, start);
+    for (int k = 0; k < 6; k++) {
+      if ((rc = graph_now(sg))) { rot_set(ctx, start); return rc; }
+      rot_set(ctx, sg->after);
+    }
+    rot_set(ctx, start);
+    return 0;
+  }
+  hipLaunchKernelGGL(k_set_tend, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, tend);
+  LAUNCHCHK(ctx);
+  for (int64_t s = 0; s < nsteps; s++) {
+    if (ctx->prof) {   // event-timed launches are not captured
+      if ((rc = step_body(ctx))) return rc;
+      continue;
+    }
+    if ((rc = graph_now(sg))) return rc;
+    HIPCHK(ctx, hipGraphLaunch(sg->exec, ctx->stream));
+    rot_set(ctx, sg->after);
+    ctx->lh_done = false;
+  }
+  return read_t(ctx, t_out);
 }
 
 // live kernel timing with HIP events on the context stream (no counterpart

@@ -34,6 +34,7 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
                                               const T* __restrict__ FY, SRd<MIX> A,
                                               SRd<MIX> U0, SWr<MIX> out, Geo g,
                                               const DevScal* __restrict__ sc, double dt_arg) {
+  if (step_off(g)) return;
   __shared__ T sm[NPL * CBS];
   const int tid = threadIdx.x;
   const long long t = xcd_block() * CBS + tid;
@@ -224,6 +225,7 @@ __global__ __launch_bounds__(BS) void k_ct(const T* __restrict__ EZX, const T* _
                                            const double* __restrict__ A, const double* __restrict__ F0,
                                            double* __restrict__ out, Geo g, const DevScal* __restrict__ sc,
                                            double dt_arg) {
+  if (step_off(g)) return;
   const long long t = xcd_block() * BS + threadIdx.x;
   const long long nx = (long long)(g.Nx + 1) * g.Ny, ny = (long long)g.Nx * (g.Ny + 1);
   if (t >= nx + ny) return;

@@ -15,6 +15,7 @@ namespace {
 template <int FT, typename T>
 __global__ __launch_bounds__(BS) void k_xface_pt(SRd<false> U, T* __restrict__ FX,
                                               T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
+  if (step_off(g)) return;
   const long long t = xcd_block() * BS + threadIdx.x;
   const long long nf = (long long)(g.Nx + 1) * g.Ny;
   if (t >= nf * 5) return;
@@ -42,6 +43,7 @@ __global__ __launch_bounds__(BS) void k_xface_pt(SRd<false> U, T* __restrict__ F
 template <int FT, typename T>
 __global__ __launch_bounds__(BS) void k_yface_pt(SRd<false> U, T* __restrict__ FY,
                                               T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
+  if (step_off(g)) return;
   const long long t = xcd_block() * BS + threadIdx.x;
   const long long nf = (long long)g.Nx * (g.Ny + 1);
   if (t >= nf * 5) return;
@@ -93,6 +95,7 @@ __device__ __forceinline__ void vertex_at(const SRd<MIX>& U, double* __restrict_
 template <int FT, bool MIX>
 __global__ __launch_bounds__(BS) void k_vertex(SRd<MIX> U, double* __restrict__ EZV, Geo g,
                                                const DevScal* __restrict__ sc) {
+  if (step_off(g)) return;
   const long long t = xcd_block() * BS + threadIdx.x;
   const long long nv = (long long)(g.Nx + 1) * (g.Ny + 1);
   if (t >= nv) return;

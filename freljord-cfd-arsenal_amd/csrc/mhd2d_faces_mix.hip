@@ -99,6 +99,7 @@ __device__ __forceinline__ void xface_at(const SRd<true>& U, T* __restrict__ FX,
 template <int FT, typename T>
 __global__ __launch_bounds__(BS) void k_xface(SRd<true> U, T* __restrict__ FX,
                                               T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
+  if (step_off(g)) return;
   const long long f = xcd_block() * BS + threadIdx.x;
   const long long nf = (long long)(g.Nx + 1) * g.Ny;
   if (f >= nf) return;
@@ -157,6 +158,7 @@ __device__ __forceinline__ void yface_at(const SRd<true>& U, T* __restrict__ FY,
 template <int FT, typename T>
 __global__ __launch_bounds__(BS) void k_yface(SRd<true> U, T* __restrict__ FY,
                                               T* __restrict__ EZ, Geo g, const DevScal* __restrict__ sc) {
+  if (step_off(g)) return;
   const long long f = xcd_block() * BS + threadIdx.x;
   const long long nf = (long long)g.Nx * (g.Ny + 1);
   if (f >= nf) return;

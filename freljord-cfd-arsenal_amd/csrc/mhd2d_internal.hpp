@@ -47,6 +47,12 @@ __device__ __forceinline__ double dunkey(long long k) {
   return __longlong_as_double(k >= 0 ? k : k ^ 0x7fffffffffffffffLL);
 }
 
+// a step that starts at t >= tend changes nothing (the reference's loop
+// test): inside mhd2d_step every kernel reads the step's active flag, set by
+// k_dt_final, through g.gate and returns at once when it is 0
+__device__ __forceinline__ bool step_off(const Geo& g) { return g.gate && *g.gate == 0; }
+__device__ __forceinline__ bool step_off(const int* gate) { return gate && *gate == 0; }
+
 // blocks of consecutive ids go to the same XCD (round-robin dispatch over 8)
 __device__ __forceinline__ long long xcd_block() {
   const unsigned b = blockIdx.x, n = gridDim.x;
@@ -121,6 +127,16 @@ enum { KC_XFACE = 0, KC_YFACE, KC_VERTEX, KC_CELL, KC_CT, KC_LIMIT, KC_SETBC, KC
 // ===========================================================================
 // host side
 
+namespace mhd2d_impl {
+// one captured step: the buffer roles it was captured for and the ones it
+// leaves (the roles of S, U0, UI, UII, T, F, F0, FI, FII)
+struct StepGraph {
+  double* before[9];
+  double* after[9];
+  hipGraphExec_t exec;
+};
+}  // namespace mhd2d_impl
+
 struct mhd2d_ctx {
   mhd2d_config cfg;
   mhd::Geo g;
@@ -152,6 +168,7 @@ struct mhd2d_ctx {
   bool prof;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_ev;
   std::vector<hipEvent_t> ev_pool;
+  std::vector<mhd2d_impl::StepGraph> graphs;   // mhd2d_step_graph: one per buffer-role phase
   char err[512];
 };
 

@@ -164,6 +164,7 @@ template <bool APPLY, bool PP, bool DIV, bool MIX>
 __global__ __launch_bounds__(BS) void k_limit(SRd<MIX> in, SWr<MIX> out,
                                               const double* __restrict__ F, const int* __restrict__ tmask,
                                               const double* __restrict__ trec, Geo g) {
+  if (step_off(g)) return;
   const long long t = xcd_block() * BS + threadIdx.x;
   const int W = g.Nx + 2;
   if (t >= (long long)W * (g.Ny + 2)) return;
@@ -211,6 +212,7 @@ __global__ __launch_bounds__(BS) void k_limit(SRd<MIX> in, SWr<MIX> out,
 template <bool MIX>
 __global__ __launch_bounds__(BS) void k_kxrcf(SRd<MIX> U, int* __restrict__ trouble, Geo g,
                                               double Ck, double hk, DevScal* sc) {
+  if (step_off(g)) return;
   const long long t = (long long)blockIdx.x * BS + threadIdx.x;
   if (t >= (long long)g.Nx * g.Ny) return;
   const int i = 1 + (int)(t % g.Nx), j = 1 + (int)(t / g.Nx);
@@ -219,7 +221,8 @@ __global__ __launch_bounds__(BS) void k_kxrcf(SRd<MIX> U, int* __restrict__ trou
   if (sc) count_flags(flag, sc);
 }
 
-__global__ void k_list_reset(DevScal* lst) {
+__global__ void k_list_reset(DevScal* lst, const int* gate) {
+  if (step_off(gate)) return;
   if (threadIdx.x == 0) lst->nlist = 0u;
 }
 
@@ -229,6 +232,7 @@ template <bool MIX>
 __global__ __launch_bounds__(BS) void k_detect(SRd<MIX> U, int* __restrict__ trouble,
                                                int* __restrict__ tmask, int* __restrict__ list, Geo g, double Ck,
                                                double hk, DevScal* sc, DevScal* lst) {
+  if (step_off(g)) return;
   const long long t = xcd_block() * BS + threadIdx.x;
   if (t >= (long long)g.Nx * g.Ny) return;
   const int i = 1 + (int)(t % g.Nx), j = 1 + (int)(t / g.Nx);
@@ -297,6 +301,7 @@ template <int IND, bool MIX>
 __global__ __launch_bounds__(BS) void k_tvb(SRd<MIX> in, int* __restrict__ trouble,
                                             int* __restrict__ tmask, double* __restrict__ trec, Geo g, double M,
                                             double Ck, double hk, DevScal* sc) {
+  if (step_off(g)) return;
   const long long ncell = (long long)g.Nx * g.Ny;
   const long long t = xcd_block() * BS + threadIdx.x;
   if (t >= ncell) return;
@@ -325,6 +330,7 @@ template <bool MIX>
 __global__ __launch_bounds__(BS) void k_tvb_list(SRd<MIX> in, const int* __restrict__ list,
                                                  int* __restrict__ tmask, double* __restrict__ trec, Geo g,
                                                  double M, const DevScal* __restrict__ lst) {
+  if (step_off(g)) return;
   const long long ncell = (long long)g.Nx * g.Ny;
   const unsigned int n = lst->nlist;
   for (unsigned int k = blockIdx.x * BS + threadIdx.x; k < n; k += gridDim.x * BS) {
@@ -376,7 +382,7 @@ int launch_tvb_list(mhd2d_ctx* ctx, const double* in) {
 // flags, tmask[] = TVB change bits of the flagged cells (0 elsewhere)
 int launch_detect_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc) {
   const Geo& g = ctx->g;
-  hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc);
+  hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, ctx->g.gate);
   int rc = with_mix(ctx, [&](auto Mc) -> int {
     constexpr bool MIX = decltype(Mc)::value;
     hipLaunchKernelGGL((k_detect<MIX>), dim3(nblk((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream,

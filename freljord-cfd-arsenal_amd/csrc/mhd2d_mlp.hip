@@ -169,6 +169,7 @@ template <bool MIX>
 __global__ __launch_bounds__(256) void k_mlp_screen(SRd<MIX> U, int* __restrict__ screen,
                                                     float* __restrict__ prob, const MlpDev* __restrict__ W,
                                                     Geo g, float l10Mh2, float l10h, float lo) {
+  if (step_off(g)) return;
   __shared__ __attribute__((aligned(16))) unsigned short sX[4][64 * XP];
   __shared__ float sZ[4][64];
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, grp = l >> 4, col = l & 15;
@@ -273,6 +274,7 @@ __device__ __forceinline__ bool candidate(const int* __restrict__ screen, const 
 __global__ __launch_bounds__(BS) void k_dilate(const int* __restrict__ screen, int* __restrict__ trouble,
                                                int* __restrict__ tmask, int* __restrict__ blist, Geo g,
                                                DevScal* lst) {
+  if (step_off(g)) return;
   const long long t = xcd_block() * BS + threadIdx.x;
   if (t >= (long long)g.Nx * g.Ny) return;
   const bool cand = candidate(screen, g, (int)(t % g.Nx), (int)(t / g.Nx));
@@ -281,7 +283,8 @@ __global__ __launch_bounds__(BS) void k_dilate(const int* __restrict__ screen, i
   band_append(cand, (int)t, blist, lst);
 }
 
-__global__ void k_reset_nlist(DevScal* lst) {
+__global__ void k_reset_nlist(DevScal* lst, const int* gate) {
+  if (step_off(gate)) return;
   if (threadIdx.x == 0) lst->nlist = 0u;
 }
 
@@ -291,6 +294,7 @@ __global__ __launch_bounds__(BS) void k_kxrcf_cands(SRd<MIX> U, int* __restrict_
                                                     int* __restrict__ tmask, int* __restrict__ list,
                                                     const int* __restrict__ blist, Geo g, double Ck, double hk,
                                                     DevScal* sc, DevScal* lst) {
+  if (step_off(g)) return;
   const unsigned int nb = lst->nband;
   for (unsigned int k0 = blockIdx.x * BS; k0 < nb; k0 += gridDim.x * BS) {
     const unsigned int k = k0 + threadIdx.x;
@@ -308,7 +312,8 @@ __global__ __launch_bounds__(BS) void k_kxrcf_cands(SRd<MIX> U, int* __restrict_
   }
 }
 
-__global__ void k_reset_nband(DevScal* lst) {
+__global__ void k_reset_nband(DevScal* lst, const int* gate) {
+  if (step_off(gate)) return;
   if (threadIdx.x == 0) lst->nband = 0u;
 }
 
@@ -523,7 +528,7 @@ int launch_mlp_screen(mhd2d_ctx* ctx, const double* in) {
   const Geo& g = ctx->g;
   int rc = launch_screen_only(ctx, in);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_reset_nband, dim3(1), dim3(64), 0, ctx->stream, ctx->sc);
+  hipLaunchKernelGGL(k_reset_nband, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, ctx->g.gate);
   hipLaunchKernelGGL(k_dilate, dim3(nb_blocks((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream,
                      (const int*)ctx->screen, ctx->trouble, ctx->tmask, ctx->blist, g, ctx->sc);
   LAUNCHCHK(ctx);
@@ -534,7 +539,7 @@ int launch_candidates_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc) {
   const Geo& g = ctx->g;
   const long long ncell = (long long)g.Nx * g.Ny;
   const unsigned nb = std::min<unsigned>(nb_blocks(ncell), 1024u);
-  hipLaunchKernelGGL(k_reset_nlist, dim3(1), dim3(64), 0, ctx->stream, ctx->sc);
+  hipLaunchKernelGGL(k_reset_nlist, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, ctx->g.gate);
   int rc = with_mix(ctx, [&](auto Mc) -> int {
     constexpr bool MIX = decltype(Mc)::value;
     hipLaunchKernelGGL((k_kxrcf_cands<MIX>), dim3(nb), dim3(BS), 0, ctx->stream, SRd<MIX>{in, g.plane}, ctx->trouble,

@@ -170,6 +170,7 @@ struct Geo {
   int pad;
   long long plane;               // doubles per plane
   double hx, hy, hx1, hy1;
+  const int* gate;               // inside mhd2d_step: the step's active flag (nullptr elsewhere)
 };
 __host__ __device__ __forceinline__ long long cix(const Geo& g, int i, int j) {
   return (long long)j * g.P + OFF + i;

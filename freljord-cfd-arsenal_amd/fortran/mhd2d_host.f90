@@ -136,6 +136,13 @@ module mhd2d_capi
       real(c_double), value :: tend
       real(c_double), intent(out) :: t_out
     end function
+    integer(c_int) function mhd2d_step_graph(ctx, tend, nsteps, t_out) bind(C, name='mhd2d_step_graph')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), value :: tend
+      integer(c_int64_t), value :: nsteps
+      real(c_double), intent(out) :: t_out
+    end function
     integer(c_int) function mhd2d_get_time(ctx, t, dt, count) bind(C, name='mhd2d_get_time')
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: ctx
@@ -363,13 +370,16 @@ subroutine load_checkpoint(path)
 end subroutine load_checkpoint
 
 ! RK3.f90:5-174 without frame output.  t, dt and count stay on the device
-! inside the loop (mhd2d_step clips dt to tend and stops a step that
-! starts at t >= tend exactly as the reference's loop test does); each
-! pass returns t so the loop condition is the reference's.  With verbose
-! the per-step line of RK3.f90:150 is printed (it costs one extra
-! reduction per step).
+! inside the loop (the step clips dt to tend, and a step that starts at
+! t >= tend changes nothing, exactly as the reference's loop test stops
+! it).  The steps run as HIP graph launches, 16 per read of t; with verbose
+! one per read, and the per-step line of RK3.f90:150 is printed (it costs
+! one extra reduction per step).
 subroutine RK3()
   use mhd2d_common
+  integer(c_int64_t) :: chunk
+  chunk = 16
+  if (verbose) chunk = 1
   call check(mhd2d_rk3_prologue(ctx), 'RK3 prologue')
   call calculate_umax()
   call calculate_totaldiv()
@@ -378,7 +388,7 @@ subroutine RK3()
   count = 0
   if (verbose) print *, t, "  ", umax, "  ", totaldiv
   do while (t < tend)
-    call check(mhd2d_step(ctx, tend, t), 'RK3 step')
+    call check(mhd2d_step_graph(ctx, tend, chunk, t), 'RK3 step')
     if (verbose) then
       call check(mhd2d_diagnostics(ctx, umax, totaldiv, pmin), 'RK3 diagnostics')
       print *, t, "  ", umax, "  ", totaldiv

@@ -134,8 +134,19 @@ int mhd2d_diagnostics(mhd2d_ctx *ctx, double *umax, double *totaldiv, double *pm
                                    /* calculate_umax/_totaldiv/_pmin.f90          */
 int mhd2d_rk3_prologue(mhd2d_ctx *ctx);                  /* RK3.f90:8-18             */
 /* one pass of the RK3.f90:135-228 loop body (dt, clip to tend, 3 stages)
-   with t and dt kept on the device; t_out (may be NULL) synchronises. */
+   with t and dt kept on the device; t_out (may be NULL) synchronises.  A
+   pass that starts at t >= tend changes nothing (the loop test of
+   RK3.f90:135), so passes may be queued without reading t back. */
 int mhd2d_step(mhd2d_ctx *ctx, double tend, double *t_out);
+/* nsteps passes of the same loop body, each replayed as one HIP graph
+   launch (captured on first use, once per phase of the six-step rotation of
+   the context's buffers; kept until mhd2d_destroy); same results as nsteps
+   calls of mhd2d_step, bit for bit.  Passes that start at t >= tend change
+   nothing, so a caller advances to tend with chunks of passes and one read
+   of t per chunk.  nsteps = 0 captures the graphs of all six phases ahead
+   and runs nothing.  While mhd2d_profile_enable is on the passes run
+   uncaptured. */
+int mhd2d_step_graph(mhd2d_ctx *ctx, double tend, int64_t nsteps, double *t_out);
 int mhd2d_get_time(mhd2d_ctx *ctx, double *t, double *dt, int64_t *count);
 int mhd2d_set_time(mhd2d_ctx *ctx, double t, int64_t count);
 /* copies of internal arrays in the reference layout (which = MHD2D_DU ..) */

@@ -1,0 +1,70 @@
+"""Graph-replayed steps (mhd2d_step_graph) and the loop test on the device.
+
+- nsteps graph launches equal nsteps eager mhd2d_step calls bit for bit, in
+  the plain form and in the mixed form with the learned screen, across more
+  than one six-step rotation of the buffer roles and for chunked replays
+- a step that starts at t >= tend changes nothing: graph chunks that run past
+  tend end on the same state, t and step count as the eager loop that stops
+  at tend, further passes leave the state untouched, and a later, larger tend
+  continues from the right buffers
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def make(pkg, prec, ind, N=48):
+    s = pkg.Solver(N, N, flux_type=pkg.HLLD, M=50.0, CFL=0.2, pp=1, indicator=ind, precision=prec, Ck=1.0)
+    if ind == pkg.IND_MLP:
+        s.mlp_set_weights(pkg.load_indicator_weights())
+    s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 7])
+    s.rk3_prologue()
+    return s
+
+
+def same(a, b):
+    ua, xa, ya = a.download()
+    ub, xb, yb = b.download()
+    return (np.array_equal(ua, ub) and np.array_equal(xa, xb) and np.array_equal(ya, yb)
+            and a.get_time() == b.get_time())
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed_nn"])
+def test_graph_equals_eager(pkg, form):
+    prec, ind = (pkg.FP64, pkg.IND_KXRCF) if form == "fp64" else (pkg.MIXED, pkg.IND_MLP)
+    a, b = make(pkg, prec, ind), make(pkg, prec, ind)
+    for _ in range(15):
+        a.step()
+    b.step_graph(0)                                # capture the six phases ahead, run nothing
+    assert b.get_time()[2] == 0
+    for k in (5, 1, 7, 2):
+        b.step_graph(k)
+    print(f"[graph] {form}: after 15 steps t = {a.get_time()[0]:.6f}")
+    assert a.get_time()[2] == 15
+    assert same(a, b)
+
+
+def test_steps_past_tend_change_nothing(pkg):
+    a, b = make(pkg, pkg.FP64, pkg.IND_KXRCF), make(pkg, pkg.FP64, pkg.IND_KXRCF)
+    tend = 0.05
+    n = 0
+    t = 0.0
+    while t < tend:
+        t = a.step(tend, sync=True)
+        n += 1
+    tb = b.step_graph(n + 9, tend, sync=True)     # 9 passes start at t = tend
+    print(f"[graph] tend {tend}: {n} steps, t = {t}, graph t = {tb}")
+    assert t == tend and tb == tend and b.get_time()[2] == n
+    assert same(a, b)
+    u0 = a.download()
+    for _ in range(4):                             # eager passes past tend
+        a.step(tend)
+    assert a.get_time()[2] == n
+    assert all(np.array_equal(x, y) for x, y in zip(u0, a.download()))
+    assert same(a, b)
+    # a larger tend continues from the right buffers, eager and graph alike
+    for _ in range(7):
+        a.step(0.08)
+    b.step_graph(7, 0.08)
+    assert same(a, b)
