@@ -569,7 +569,7 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
   hipError_t e = hipGetDevice(&ctx->device);
   if (e != hipSuccess) { free(ctx); return -2; }
   const size_t sbytes = (size_t)state_bytes(cfg->precision == MHD2D_MIXED, g.plane);
-  const size_t fbytes = (size_t)6 * g.plane * sizeof(double);
+  const size_t fbytes = (size_t)face_planes(ctx) * g.plane * sizeof(double);
   const size_t xbytes = (size_t)40 * g.plane * sizeof(double);
   const size_t ebytes = (size_t)5 * g.plane * sizeof(double);
   ctx->nblk_cells = nblk((long long)g.Nx * g.Ny);
@@ -642,6 +642,16 @@ int mhd2d_sync(mhd2d_ctx* ctx) {
 }
 
 // host uh(0:Nx+1,0:Ny+1,10,8) <-> device planes; Bx(0:Nx,1:Ny,3), By(1:Nx,0:Ny,3)
+// mixed form: faces written from outside the CT update start their
+// double-double low parts again at zero
+static int clear_face_lo(mhd2d_ctx* ctx, double* F, bool x, bool y) {
+  if (ctx->cfg.precision != MHD2D_MIXED) return 0;
+  const size_t pb = (size_t)ctx->g.plane * sizeof(double);
+  if (x) HIPCHK(ctx, hipMemsetAsync(F + FACE_LO_X * ctx->g.plane, 0, pb, ctx->stream));
+  if (y) HIPCHK(ctx, hipMemsetAsync(F + FACE_LO_Y * ctx->g.plane, 0, pb, ctx->stream));
+  return 0;
+}
+
 int mhd2d_upload(mhd2d_ctx* ctx, const double* uh, const double* Bx, const double* By) {
   int rc = ctx_check(ctx);
   if (rc) return rc;
@@ -669,6 +679,7 @@ int mhd2d_upload(mhd2d_ctx* ctx, const double* uh, const double* Bx, const doubl
     for (int k = 0; k < 3; k++)
       HIPCHK(ctx, hipMemcpy2DAsync(ctx->F + (3 + k) * g.plane + cix(g, 1, 0), P * 8, By + (size_t)g.Nx * (g.Ny + 1) * k,
                                    g.Nx * 8, g.Nx * 8, g.Ny + 1, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = clear_face_lo(ctx, ctx->F, Bx != nullptr, By != nullptr))) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->lh_done = false;
   return 0;
@@ -789,6 +800,7 @@ int mhd2d_init_data(mhd2d_ctx* ctx, int32_t ic, const double* extra, int32_t nex
   hipLaunchKernelGGL(k_init_faces, dim3(nblk(nf)), dim3(BS), 0, ctx->stream, ctx->F, g, ctx->cfg.xa, ctx->cfg.ya,
                      (const ICPar*)ctx->icp);
   LAUNCHCHK(ctx);
+  if ((rc = clear_face_lo(ctx, ctx->F, true, true))) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->lh_done = false;
   return 0;
@@ -798,6 +810,28 @@ int mhd2d_init_data(mhd2d_ctx* ctx, int32_t ic, const double* extra, int32_t nex
 // checkpoint (format in include/mhd2d.h)
 }  // extern "C"
 namespace {
+
+// the mixed form's face low parts (Bx mode 1 on x faces, then By mode 1 on
+// y faces, Fortran order) between host and the current face buffer
+int face_lo_io(mhd2d_ctx* ctx, double* host, bool to_device) {
+  const Geo& g = ctx->g;
+  const size_t P = g.P, nx = (size_t)(g.Nx + 1) * g.Ny;
+  double* dx = ctx->F + FACE_LO_X * g.plane + cix(g, 0, 1);
+  double* dy = ctx->F + FACE_LO_Y * g.plane + cix(g, 1, 0);
+  if (to_device) {
+    HIPCHK(ctx, hipMemcpy2DAsync(dx, P * 8, host, (g.Nx + 1) * 8, (g.Nx + 1) * 8, g.Ny, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    HIPCHK(ctx, hipMemcpy2DAsync(dy, P * 8, host + nx, g.Nx * 8, g.Nx * 8, g.Ny + 1, hipMemcpyHostToDevice,
+                                 ctx->stream));
+  } else {
+    HIPCHK(ctx, hipMemcpy2DAsync(host, (g.Nx + 1) * 8, dx, P * 8, (g.Nx + 1) * 8, g.Ny, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    HIPCHK(ctx, hipMemcpy2DAsync(host + nx, g.Nx * 8, dy, P * 8, g.Nx * 8, g.Ny + 1, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
 
 struct CkHeader {
   char magic[8];
@@ -863,8 +897,11 @@ int mhd2d_checkpoint_save(mhd2d_ctx* ctx, const char* path) {
   const Geo& g = ctx->g;
   const size_t W = g.Nx + 2, H = g.Ny + 2;
   const size_t nbx = (size_t)(g.Nx + 1) * g.Ny * 3, nby = (size_t)g.Nx * (g.Ny + 1) * 3;
-  std::vector<double> uh(W * H * 80), pay(52 * W * H + nbx + nby);
+  const bool lo = ctx->cfg.precision == MHD2D_MIXED;   // version 2: the faces' low parts follow
+  const size_t nlo = lo ? (nbx + nby) / 3 : 0;
+  std::vector<double> uh(W * H * 80), pay(52 * W * H + nbx + nby + nlo);
   if ((rc = mhd2d_download(ctx, uh.data(), pay.data() + 52 * W * H, pay.data() + 52 * W * H + nbx))) return rc;
+  if (lo && (rc = face_lo_io(ctx, pay.data() + 52 * W * H + nbx + nby, false))) return rc;
   int bn[52][2];
   plane_list(bn);
   for (int p = 0; p < 52; p++)
@@ -872,7 +909,7 @@ int mhd2d_checkpoint_save(mhd2d_ctx* ctx, const char* path) {
   CkHeader h;
   memset(&h, 0, sizeof h);
   memcpy(h.magic, "MHD2DCK1", 8);
-  h.version = 1;
+  h.version = lo ? 2 : 1;
   h.header_bytes = sizeof h;
   h.Nx = g.Nx; h.Ny = g.Ny; h.degree = 2; h.nplanes = 52;
   h.flux_type = ctx->cfg.flux_type; h.precision = ctx->cfg.precision;
@@ -906,9 +943,11 @@ int mhd2d_checkpoint_load(mhd2d_ctx* ctx, const char* path) {
   uint32_t crc_file = 0;
   const char* why = nullptr;
   if (fread(&h, sizeof h, 1, f) != 1) why = "short header";
-  else if (memcmp(h.magic, "MHD2DCK1", 8) || h.version != 1 || h.header_bytes != sizeof h) why = "not a version-1 checkpoint";
+  else if (memcmp(h.magic, "MHD2DCK1", 8) || (h.version != 1 && h.version != 2) || h.header_bytes != sizeof h)
+    why = "not a version-1 or version-2 checkpoint";
   else if (h.Nx != g.Nx || h.Ny != g.Ny || h.degree != 2 || h.nplanes != 52) why = "grid differs from the context";
-  else if (h.payload_bytes != (52 * W * H + nbx + nby) * sizeof(double)) why = "payload size differs";
+  else if (h.payload_bytes != (52 * W * H + nbx + nby + (h.version == 2 ? (nbx + nby) / 3 : 0)) * sizeof(double))
+    why = "payload size differs";
   if (!why) {
     pay.resize(h.payload_bytes / sizeof(double));
     if (fread(pay.data(), 1, h.payload_bytes, f) != h.payload_bytes || fread(&crc_file, 4, 1, f) != 1)
@@ -925,6 +964,9 @@ int mhd2d_checkpoint_load(mhd2d_ctx* ctx, const char* path) {
   for (int p = 0; p < 52; p++)
     memcpy(uh.data() + W * H * (bn[p][0] + 10 * bn[p][1]), pay.data() + p * W * H, W * H * sizeof(double));
   if ((rc = mhd2d_upload(ctx, uh.data(), pay.data() + 52 * W * H, pay.data() + 52 * W * H + nbx))) return rc;
+  if (h.version == 2 && ctx->cfg.precision == MHD2D_MIXED &&
+      (rc = face_lo_io(ctx, pay.data() + 52 * W * H + nbx + nby, true)))
+    return rc;
   return mhd2d_set_time(ctx, h.t, h.count);
 }
 
@@ -963,6 +1005,7 @@ int mhd2d_calculate_az(mhd2d_ctx* ctx) {
   const long long nf = (long long)(g.Nx + 1) * g.Ny + (long long)g.Nx * (g.Ny + 1);
   hipLaunchKernelGGL(k_calc_az, dim3(nblk(nf)), dim3(BS), 0, ctx->stream, ctx->F, g, ctx->cfg.xa, ctx->cfg.ya);
   LAUNCHCHK(ctx);
+  if ((rc = clear_face_lo(ctx, ctx->F, true, true))) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->lh_done = false;
   return 0;
@@ -1053,6 +1096,10 @@ int mhd2d_rk_stage(mhd2d_ctx* ctx, int32_t stage, double dt) {
     return fail(ctx, -1, "rk_stage: stage must be 1, 2 or 3");
   }
   LAUNCHCHK(ctx);
+  // the subroutine path combines the faces in plain FP64 (as the reference);
+  // the mixed form's low parts are not carried through it
+  for (double* fb : {ctx->F, ctx->F0, ctx->FI, ctx->FII})
+    if ((rc = clear_face_lo(ctx, fb, true, true))) return rc;
   ctx->lh_done = false;
   return 0;
 }
@@ -1230,7 +1277,7 @@ int step_body(mhd2d_ctx* ctx) {
   if ((rc = launch_limit(ctx, ctx->T, ctx->UI, ctx->F, true, pp, true, sc, 3))) return rc;
   // a switched-off step: the start-of-step state and faces become the new ones
   const long long ns = (long long)(state_bytes(ctx->cfg.precision == MHD2D_MIXED, ctx->g.plane) / sizeof(double));
-  const long long nf = 6LL * ctx->g.plane;
+  const long long nf = (long long)face_planes(ctx) * ctx->g.plane;
   const unsigned nbc = std::min<unsigned>(nblk(ns), 1024u);
   hipLaunchKernelGGL(k_carry, dim3(nbc), dim3(BS), 0, ctx->stream, (const double*)ctx->U0, ctx->UI, ns, ctx->g.gate);
   hipLaunchKernelGGL(k_carry, dim3(nbc), dim3(BS), 0, ctx->stream, (const double*)ctx->F0, ctx->F, nf, ctx->g.gate);

@@ -237,6 +237,7 @@ __global__ __launch_bounds__(BS) void k_ct(const T* __restrict__ EZX, const T* _
   double d[3];
   long long o;
   int base;
+  double va, vb, ch;   // mode-1 increment dt * (va - vb) / h of the face (mixed form)
   if (t < nx) {
     const int i = (int)(t % (g.Nx + 1)), j = 1 + (int)(t / (g.Nx + 1));
     o = cix(g, i, j);
@@ -257,6 +258,7 @@ __global__ __launch_bounds__(BS) void k_ct(const T* __restrict__ EZX, const T* _
     const double vU = EZV[cix(g, i, j)], vD = EZV[cix(g, i, j - 1)];
 #pragma unroll
     for (int k = 0; k < 3; k++) d[k] = (a[k] - vU * EzP[k] / g.hy + vD * EzM[k] / g.hy) / mmE[k];
+    va = vD; vb = vU; ch = g.hy;
   } else {
     const long long u = t - nx;
     const int i = 1 + (int)(u % g.Nx), j = (int)(u / g.Nx);
@@ -278,6 +280,7 @@ __global__ __launch_bounds__(BS) void k_ct(const T* __restrict__ EZX, const T* _
     const double vR = EZV[cix(g, i, j)], vL = EZV[cix(g, i - 1, j)];
 #pragma unroll
     for (int k = 0; k < 3; k++) d[k] = (a[k] + vR * EzP[k] / g.hx - vL * EzM[k] / g.hx) / mmE[k];
+    va = vR; vb = vL; ch = g.hx;
   }
 #pragma unroll
   for (int k = 0; k < 3; k++) {
@@ -287,7 +290,27 @@ __global__ __launch_bounds__(BS) void k_ct(const T* __restrict__ EZX, const T* _
     else if (MODE == 1) r = F[oo] + dt * d[k];
     else if (MODE == 2) r = (3.0 / 4.0) * F0[oo] + (1.0 / 4.0) * (A[oo] + dt * d[k]);
     else r = (1.0 / 3.0) * F0[oo] + (2.0 / 3.0) * (A[oo] + dt * d[k]);
+    if (sizeof(T) == 4 && MODE > 0 && k == 0) continue;   // mixed form: mode 1 below
     out[oo] = r;
+  }
+  if (sizeof(T) == 4 && MODE > 0) {
+    // mixed form, mode 1 in double-double: the increment dt/h (va - vb) with
+    // the vertex difference taken exactly (-dt/hy (Ez_U - Ez_D) on x faces,
+    // dt/hx (Ez_R - Ez_L) on y faces), so the four increments of a cell
+    // cancel in its discrete divergence and the RK combinations keep the
+    // cancellation; the stored high parts carry one rounding, not one per stage
+    const long long oo = base * pl + o, ol = (base == 0 ? FACE_LO_X : FACE_LO_Y) * pl + o;
+    const dd inc = dd_scale(dt / ch, two_sum(va, -vb));
+    dd r;
+    if (MODE == 1) {
+      r = dd_add(dd{F[oo], F[ol]}, inc);
+    } else {
+      const dd x = dd_add(dd{A[oo], A[ol]}, inc), f0{F0[oo], F0[ol]};
+      r = MODE == 2 ? dd_add(dd_scale(3.0 / 4.0, f0), dd_scale(1.0 / 4.0, x))
+                    : dd_add(dd_scale(1.0 / 3.0, f0), dd_scale(2.0 / 3.0, x));
+    }
+    out[oo] = r.hi;
+    out[ol] = r.lo;
   }
 }
 

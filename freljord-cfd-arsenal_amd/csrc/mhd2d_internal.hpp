@@ -119,6 +119,13 @@ constexpr int MLP_NPARAM = MLP_OFF_B3 + 1;           // 8129
 // y faces); EMF planes 0 and 1 hold its L_0 and L_1 moments
 constexpr int FM_N = 3, FM_P3 = 24;
 
+// face buffers: planes 0..2 Bx modes on x faces, 3..5 By modes on y faces;
+// the mixed form adds the low parts of the double-double mode-1 values
+// (plane FACE_LO_X: Bx on x faces, FACE_LO_Y: By on y faces) that its CT
+// update carries, so that the discrete divergence of the faces does not
+// accumulate rounding from step to step
+constexpr int FACE_PL = 6, FACE_PL_MIX = 8, FACE_LO_X = 6, FACE_LO_Y = 7;
+
 // kernel classes for mhd2d_profile_get
 enum { KC_XFACE = 0, KC_YFACE, KC_VERTEX, KC_CELL, KC_CT, KC_LIMIT, KC_SETBC, KC_DT, KC_KXRCF, KC_DIAG, KC_N };
 
@@ -173,6 +180,8 @@ struct mhd2d_ctx {
 };
 
 namespace mhd2d_impl {
+
+inline int face_planes(const mhd2d_ctx* ctx) { return ctx->cfg.precision == MHD2D_MIXED ? FACE_PL_MIX : FACE_PL; }
 
 inline int fail(mhd2d_ctx* ctx, int code, const char* fmt, ...) {
   if (ctx) {

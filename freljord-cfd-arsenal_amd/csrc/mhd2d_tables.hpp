@@ -176,6 +176,34 @@ __host__ __device__ __forceinline__ long long cix(const Geo& g, int i, int j) {
   return (long long)j * g.P + OFF + i;
 }
 
+// error-free transformations and double-double arithmetic (Dekker; Knuth's
+// two-sum); the FMA gives the exact product error.  Contraction stays off:
+// a product fused into a later sum would break the error terms.
+struct dd {
+  double hi, lo;
+};
+__host__ __device__ __forceinline__ dd two_sum(double a, double b) {
+  MHD_NOCONTRACT
+  const double s = a + b, bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+__host__ __device__ __forceinline__ dd fast_two_sum(double a, double b) {   // |a| >= |b|
+  MHD_NOCONTRACT
+  const double s = a + b;
+  return {s, b - (s - a)};
+}
+__host__ __device__ __forceinline__ dd dd_add(dd a, dd b) {
+  MHD_NOCONTRACT
+  const dd s = two_sum(a.hi, b.hi);
+  return fast_two_sum(s.hi, s.lo + (a.lo + b.lo));
+}
+__host__ __device__ __forceinline__ dd dd_scale(double c, dd a) {   // c * a
+  MHD_NOCONTRACT
+  const double p = c * a.hi;
+  const double e = fma(c, a.hi, -p);
+  return fast_two_sum(p, fma(c, a.lo, e));
+}
+
 // the one k with trk(side, b, k) != 0
 __host__ __device__ constexpr int trace_k(int side, int b) {
   return trk(side, b, 0) != 0.0 ? 0 : trk(side, b, 1) != 0.0 ? 1 : trk(side, b, 2) != 0.0 ? 2 : 3;

@@ -1,10 +1,13 @@
 """Precision study of the mixed form on the smooth MHD vortex.
 
-    python freljord-cfd-arsenal_amd/precision_study.py [--n 256 512 1024 2048] [--tend 1.0] [--out study.json]
+    python freljord-cfd-arsenal_amd/precision_study.py [--n 256 512 1024 2048] [--tend 1.0]
+        [--domain XA XB YA YB] [--out study.json]
 
 The vortex of init1.txt (rho = 1, p = 1, u_inf = (1, 1), magnetic
-perturbation of amplitude 1, gamma = 5/3, [-5, 5]^2 periodic) is advanced
-with HLLD, no limiter and CFL 0.15 (calculate_dt.f90:8) to tend at each
+perturbation of amplitude 1, gamma = 5/3, periodic; the domain of
+init1.txt:41-44 is [-10, 10]^2, --domain overrides it) is advanced with
+HLLD, TVB on every cell with M = 1e6 (init1.txt:55, which leaves this smooth
+flow unlimited) and CFL 0.15 (calculate_dt.f90:8) to tend at each
 resolution, once in plain FP64 and once in the mixed form.  Reported per
 resolution:
 - the L2 error of each form against the exact solution (calculate_L2error,
@@ -56,11 +59,12 @@ def field_rel_l2(a, b):
     return out
 
 
-def run_vortex(N, precision, tend, chunk=0.1, log=None):
+def run_vortex(N, precision, tend, chunk=0.1, log=None, domain=None):
     """advance the vortex to tend; returns (modal array, L2 errors, steps, seconds, div B)"""
     P = _pkg()
     d = P.ic_defaults(P.IC_VORTEX)
-    s = P.Solver(N, N, flux_type=P.HLLD, bc=d["bc"], domain=d["domain"], M=d["M"], CFL=0.15, precision=precision)
+    s = P.Solver(N, N, flux_type=P.HLLD, bc=d["bc"], domain=tuple(domain or d["domain"]), M=d["M"], CFL=0.15,
+                 pp=0, indicator=P.IND_ALL, precision=precision)
     try:
         s.init_data(P.IC_VORTEX)
         s.sync()
@@ -82,12 +86,12 @@ def run_vortex(N, precision, tend, chunk=0.1, log=None):
         s.close()
 
 
-def study(Ns, tend, chunk=0.1, log=None):
+def study(Ns, tend, chunk=0.1, log=None, domain=None):
     P = _pkg()
     rows = []
     for N in Ns:
-        u64, e64, n64, s64, d64 = run_vortex(N, P.FP64, tend, chunk, log)
-        umx, emx, nmx, smx, dmx = run_vortex(N, P.MIXED, tend, chunk, log)
+        u64, e64, n64, s64, d64 = run_vortex(N, P.FP64, tend, chunk, log, domain)
+        umx, emx, nmx, smx, dmx = run_vortex(N, P.MIXED, tend, chunk, log, domain)
         rows.append({"N": N, "steps": n64, "steps_mixed": nmx, "sec_fp64": s64, "sec_mixed": smx,
                      "L2_fp64": [float(x) for x in e64], "L2_mixed": [float(x) for x in emx],
                      "mixed_vs_fp64": field_rel_l2(umx, u64), "divB_fp64": d64, "divB_mixed": dmx})
@@ -104,8 +108,8 @@ def study(Ns, tend, chunk=0.1, log=None):
     return rows
 
 
-def markdown(rows, tend):
-    out = [f"Smooth MHD vortex, HLLD, CFL 0.15, t = {tend}: L2 error of rho against the exact solution, "
+def markdown(rows, tend, domain):
+    out = [f"Smooth MHD vortex on {domain}, HLLD, CFL 0.15, t = {tend}: L2 error of rho against the exact solution, "
            "convergence order, and the mixed form against FP64 (relative L2, every field)", "",
            "| N | steps | L2 rho FP64 | order | L2 rho mixed | order | " + " | ".join(FIELDS) + " | div B mixed | s FP64 | s mixed |",
            "|---|---|---|---|---|---|" + "---|" * len(FIELDS) + "---|---|---|"]
@@ -124,13 +128,16 @@ def main():
     ap.add_argument("--n", type=int, nargs="+", default=[256, 512, 1024, 2048])
     ap.add_argument("--tend", type=float, default=1.0)
     ap.add_argument("--chunk", type=float, default=0.1, help="time between progress lines")
+    ap.add_argument("--domain", type=float, nargs=4, metavar=("XA", "XB", "YA", "YB"),
+                    help="periodic domain (default: init1.txt's [-10, 10]^2)")
     ap.add_argument("--out", help="write the rows as JSON")
     a = ap.parse_args()
-    rows = study(a.n, a.tend, a.chunk, log=lambda m: print(m, flush=True))
-    print(markdown(rows, a.tend), flush=True)
+    dom = tuple(a.domain) if a.domain else _pkg().ic_defaults(_pkg().IC_VORTEX)["domain"]
+    rows = study(a.n, a.tend, a.chunk, log=lambda m: print(m, flush=True), domain=dom)
+    print(markdown(rows, a.tend, dom), flush=True)
     if a.out:
         with open(a.out, "w") as f:
-            json.dump({"tend": a.tend, "rows": rows}, f, indent=1)
+            json.dump({"tend": a.tend, "domain": dom, "rows": rows}, f, indent=1)
 
 
 if __name__ == "__main__":

@@ -98,17 +98,23 @@ int mhd2d_init_data(mhd2d_ctx *ctx, int32_t ic, const double *extra, int32_t nex
    uh, the face fields Bx/By and (t, count), so that a run restarts from
    it bit for bit (the reference writes only uh and has no restart).
    Little-endian file = 128-byte header, payload, CRC-32 trailer:
-     header  "MHD2DCK1", u32 version 1, u32 header bytes 128, i32 Nx, Ny,
+     header  "MHD2DCK1", u32 version (1; 2 from a mixed-form context),
+             u32 header bytes 128, i32 Nx, Ny,
              polynomial degree 2, planes 52, flux_type, precision, bcR, bcL,
              bcU, bcD, then f64 xa, xb, ya, yb, t, dt, i64 count,
              u64 payload bytes, zero padding
      payload the 52 active (mode, variable) planes of uh(0:Nx+1, 0:Ny+1, :, :)
              in the order mode 1-6 of variables 1-8, then (7, Bx), (9, Bx),
              (8, By), (10, By), each plane i fastest; then Bx(0:Nx, 1:Ny, 3)
-             and By(1:Nx, 0:Ny, 3) in Fortran order (all f64)
+             and By(1:Nx, 0:Ny, 3) in Fortran order (all f64); version 2
+             appends the low parts of the double-double mode-1 face values
+             the mixed form's CT update carries: Bx(0:Nx, 1:Ny) then
+             By(1:Nx, 0:Ny), Fortran order
      trailer u32 CRC-32 (IEEE 802.3, as zlib.crc32) of header + payload
    load checks magic, version, Nx, Ny, size and CRC before touching the
-   context, then uploads the state and sets (t, count). */
+   context, then uploads the state and sets (t, count); a version-1 file
+   gives a mixed-form context zero low parts, and a plain context ignores
+   the low parts of a version-2 file. */
 int mhd2d_checkpoint_save(mhd2d_ctx *ctx, const char *path);
 int mhd2d_checkpoint_load(mhd2d_ctx *ctx, const char *path);
 /* defaults of one initial condition (initN.txt); -1 for an unknown ic */

@@ -68,3 +68,22 @@ def test_steps_past_tend_change_nothing(pkg):
         a.step(0.08)
     b.step_graph(7, 0.08)
     assert same(a, b)
+
+
+def test_mixed_divergence_does_not_accumulate(pkg):
+    """the plain form's div B is a random walk of the face roundings (as in
+    the reference: it grows as the square root of the step count); the mixed
+    form's double-double CT update keeps it at one rounding"""
+    out = {}
+    for name, prec in (("fp64", pkg.FP64), ("mixed", pkg.MIXED)):
+        s = make(pkg, prec, pkg.IND_KXRCF, N=128)
+        div = []
+        for k in (10, 90, 300):
+            s.step_graph(k)
+            div.append(s.diagnostics()["totaldiv"])
+        out[name] = div
+        s.close()
+    print(f"[graph] div B after 10, 100, 400 steps: fp64 {out['fp64']}, mixed {out['mixed']}")
+    assert out["fp64"][2] > 3 * out["fp64"][0]
+    assert out["mixed"][2] < 2 * out["mixed"][0]
+    assert out["mixed"][2] < 0.25 * out["fp64"][2]

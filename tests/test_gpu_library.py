@@ -127,3 +127,33 @@ def test_checkpoint_format(pkg, tmp_path):
     other = pkg.Solver(16, 16)
     with pytest.raises(pkg.MHD2DError, match="grid"):
         other.load_checkpoint(str(path))
+
+
+def test_checkpoint_version2_mixed(pkg, tmp_path):
+    """a mixed-form context writes version 2: the v1 payload, then the low
+    parts of the double-double mode-1 face values, each at most half an ulp
+    of its high part; a plain context reads the file and ignores them"""
+    path = tmp_path / "ck2.bin"
+    s = _run(pkg, pkg.MIXED, pkg.IND_KXRCF, 3)
+    s.save_checkpoint(str(path))
+    raw = path.read_bytes()
+    N = 32
+    W = H = N + 2
+    n1 = 52 * W * H + 2 * 3 * (N + 1) * N
+    pay = (n1 + 2 * (N + 1) * N) * 8
+    assert len(raw) == 128 + pay + 4
+    assert struct.unpack_from("<I", raw, 8)[0] == 2
+    assert struct.unpack_from("<Q", raw, 112)[0] == pay
+    assert struct.unpack_from("<I", raw, 128 + pay)[0] == zlib.crc32(raw[:128 + pay])
+    _, Bx, By = s.download()
+    lo = np.frombuffer(raw, dtype="<f8", count=2 * (N + 1) * N, offset=128 + n1 * 8)
+    lx = lo[:(N + 1) * N].reshape((N + 1, N), order="F")
+    ly = lo[(N + 1) * N:].reshape((N, N + 1), order="F")
+    assert np.all(np.abs(lx) <= 0.5 * np.spacing(np.abs(Bx[:, :, 0])))
+    assert np.all(np.abs(ly) <= 0.5 * np.spacing(np.abs(By[:, :, 0])))
+    assert np.abs(lo).max() > 0          # the CT update has carried low parts
+    p = pkg.Solver(32, 32, flux_type=pkg.HLLD, M=50.0, CFL=0.2, pp=1, indicator=pkg.IND_KXRCF, Ck=1.0)
+    p.load_checkpoint(str(path))
+    uq, qx, qy = p.download()
+    assert np.array_equal(qx, Bx) and np.array_equal(qy, By)
+    assert p.get_time()[2] == 3
