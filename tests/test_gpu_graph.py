@@ -87,3 +87,21 @@ def test_mixed_divergence_does_not_accumulate(pkg):
     assert out["fp64"][2] > 3 * out["fp64"][0]
     assert out["mixed"][2] < 2 * out["mixed"][0]
     assert out["mixed"][2] < 0.25 * out["fp64"][2]
+
+
+def test_graph_sees_new_weights(pkg):
+    """the captured graphs read the indicator weights through the context's
+    weight buffer: weights set after the capture take effect in the replays"""
+    a, b = make(pkg, pkg.MIXED, pkg.IND_MLP), make(pkg, pkg.MIXED, pkg.IND_MLP)
+    b.step_graph(0)
+    params = pkg.load_indicator_weights()
+    other = params.copy()
+    other[-1] += 50.0                   # output bias: the screen passes every cell
+    for s in (a, b):
+        s.mlp_set_weights(other)
+    for _ in range(4):
+        a.step()
+    b.step_graph(4)
+    assert same(a, b)
+    pa, pb = a.mlp_probability(), b.mlp_probability()
+    assert np.array_equal(pa, pb) and pa.min() > 0.5
