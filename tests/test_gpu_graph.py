@@ -92,16 +92,17 @@ def test_mixed_divergence_does_not_accumulate(pkg):
 def test_graph_sees_new_weights(pkg):
     """the captured graphs read the indicator weights through the context's
     weight buffer: weights set after the capture take effect in the replays"""
-    a, b = make(pkg, pkg.MIXED, pkg.IND_MLP), make(pkg, pkg.MIXED, pkg.IND_MLP)
+    a, b, c = (make(pkg, pkg.MIXED, pkg.IND_MLP) for _ in range(3))
     b.step_graph(0)
-    params = pkg.load_indicator_weights()
-    other = params.copy()
-    other[-1] += 50.0                   # output bias: the screen passes every cell
+    other = pkg.load_indicator_weights().copy()
+    other[-1] += 50.0                   # output bias: the screen passes most cells
     for s in (a, b):
         s.mlp_set_weights(other)
     for _ in range(4):
         a.step()
+        c.step()                        # the trained weights
     b.step_graph(4)
     assert same(a, b)
-    pa, pb = a.mlp_probability(), b.mlp_probability()
-    assert np.array_equal(pa, pb) and pa.min() > 0.5
+    pa, pb, pc = a.mlp_probability(), b.mlp_probability(), c.mlp_probability()
+    assert np.array_equal(pa, pb)
+    assert (pa > 0.5).mean() > 0.5 > (pc > 0.5).mean()
