@@ -52,12 +52,14 @@ def alg_bytes(form):
         "vertex": S + 8,              # state in; one FP64 vertex EMF out
         # state + x/y fluxes in, stage value out; the RK inputs average
         # (0 + 2 + 2) / 3 state reads (stages 2 and 3 read U0 and the
-        # unlimited previous stage value)
-        "cell": S + 2 * NF * B + S + 4 * S / 3.0,
+        # unlimited previous stage value); the plain form's cell kernel also
+        # runs the limiter pass without TVB (new faces in, limited value out)
+        "cell": S + 2 * NF * B + S + 4 * S / 3.0 + (S + 48 if form == "fp64" else 0),
         "ct": 2 * NE * B + 8 + FB + FB + 4 * FB / 3.0,   # face + vertex EMFs, faces in/out, RK inputs
         # limiter pass: the detector/TVB kernel reads the state (KXRCF
         # traces), the apply pass reads state + faces and writes the state
-        "limit": S + S + 48 + S + 8,
+        # (plain form: only the detector; the TVB-changed cells are sparse)
+        "limit": S + 8 if form == "fp64" else S + S + 48 + S + 8,
         "setbc": 0.0,
         "dt": 64,                       # 8 cell averages
         "kxrcf": S,

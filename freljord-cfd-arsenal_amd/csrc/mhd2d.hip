@@ -1252,29 +1252,39 @@ int step_body(mhd2d_ctx* ctx) {
   if ((rc = launch_dt(ctx, 1))) return rc;
   DevScal* sc = ctx->sc;
   const bool pp = ctx->cfg.pp != 0;
+  // one stage: fluxes of S; the CT update of the faces (Fa, Fb -> Fo); the
+  // cell update (A, U0 -> out); the limiter pass (out -> lim, faces Fo).
+  // In the plain form, except with TVB on every cell, the limiter pass
+  // without TVB runs in the cell kernel's epilogue (the CT goes first so the
+  // new faces are there), and only the cells TVB changes are limited again.
+  const bool fused = ctx->cfg.indicator != MHD2D_IND_ALL && ctx->cfg.precision == MHD2D_FP64;
+  auto stage = [&](int k, const double* Fa, const double* Fb, double* Fo, const double* A, double* out,
+                   double* lim) -> int {
+    int r;
+    if ((r = launch_fluxes(ctx, ctx->S, true, sc))) return r;
+    if (fused) {
+      if ((r = launch_ct(ctx, k, k == 1 ? Fa : nullptr, k == 1 ? nullptr : Fa, Fb, Fo, sc, 0.0))) return r;
+      if ((r = launch_cell(ctx, k, ctx->S, A, ctx->U0, out, sc, 0.0, lim, Fo, pp))) return r;
+      return launch_limit_fixup(ctx, out, lim, Fo, pp, sc, k);
+    }
+    if ((r = launch_cell(ctx, k, ctx->S, A, ctx->U0, out, sc, 0.0))) return r;
+    if ((r = launch_ct(ctx, k, k == 1 ? Fa : nullptr, k == 1 ? nullptr : Fa, Fb, Fo, sc, 0.0))) return r;
+    return launch_limit(ctx, out, lim, Fo, true, pp, true, sc, k);
+  };
   // stage 1 (RK3.f90:151-174): set_bc; UI = S + dt L(S); S1 = limit(UI);
   // U0 keeps S with its fresh halos
   if ((rc = launch_setbc(ctx, ctx->S, sc))) return rc;
-  if ((rc = launch_fluxes(ctx, ctx->S, true, sc))) return rc;
-  if ((rc = launch_cell(ctx, 1, ctx->S, nullptr, nullptr, ctx->UI, sc, 0.0))) return rc;
-  if ((rc = launch_ct(ctx, 1, ctx->F, nullptr, nullptr, ctx->FI, sc, 0.0))) return rc;
-  if ((rc = launch_limit(ctx, ctx->UI, ctx->T, ctx->FI, true, pp, true, sc, 1))) return rc;
+  if ((rc = stage(1, ctx->F, nullptr, ctx->FI, nullptr, ctx->UI, ctx->T))) return rc;
   if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;          // RK3.f90:174
   std::swap(ctx->U0, ctx->S);   // U0 <- start-of-step state
   std::swap(ctx->S, ctx->T);    // S  <- limited stage-1 state
   std::swap(ctx->F0, ctx->F);   // F0 <- start-of-step faces; F is free
   // stage 2: UII = 3/4 U0 + 1/4 (UI + dt L(S)); S2 = limit(UII)
-  if ((rc = launch_fluxes(ctx, ctx->S, true, sc))) return rc;
-  if ((rc = launch_cell(ctx, 2, ctx->S, ctx->UI, ctx->U0, ctx->UII, sc, 0.0))) return rc;
-  if ((rc = launch_ct(ctx, 2, nullptr, ctx->FI, ctx->F0, ctx->FII, sc, 0.0))) return rc;
-  if ((rc = launch_limit(ctx, ctx->UII, ctx->T, ctx->FII, true, pp, true, sc, 2))) return rc;
+  if ((rc = stage(2, ctx->FI, ctx->F0, ctx->FII, ctx->UI, ctx->UII, ctx->T))) return rc;
   if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;          // RK3.f90:200 (stage 3 set_bc)
   std::swap(ctx->S, ctx->T);
   // stage 3: T = 1/3 U0 + 2/3 (UII + dt L(S)); S_new = limit(T)
-  if ((rc = launch_fluxes(ctx, ctx->S, true, sc))) return rc;
-  if ((rc = launch_cell(ctx, 3, ctx->S, ctx->UII, ctx->U0, ctx->T, sc, 0.0))) return rc;
-  if ((rc = launch_ct(ctx, 3, nullptr, ctx->FII, ctx->F0, ctx->F, sc, 0.0))) return rc;
-  if ((rc = launch_limit(ctx, ctx->T, ctx->UI, ctx->F, true, pp, true, sc, 3))) return rc;
+  if ((rc = stage(3, ctx->FII, ctx->F0, ctx->F, ctx->UII, ctx->T, ctx->UI))) return rc;
   // a switched-off step: the start-of-step state and faces become the new ones
   const long long ns = (long long)(state_bytes(ctx->cfg.precision == MHD2D_MIXED, ctx->g.plane) / sizeof(double));
   const long long nf = (long long)face_planes(ctx) * ctx->g.plane;

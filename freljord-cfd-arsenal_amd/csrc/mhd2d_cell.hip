@@ -3,6 +3,7 @@
 // transport update of the face B modes, and the elementwise stage update of
 // the per-subroutine API.
 #include "mhd2d_internal.hpp"
+#include "mhd2d_limit_cell.hpp"
 
 namespace mhd2d_impl {
 namespace {
@@ -29,10 +30,19 @@ namespace {
 // algorithmic traffic).  In the plain form the epilogue takes S from LDS too.
 // mixed form (T = float) on square cells (SQ): the volume fluxes are not
 // scaled by 1/hx1, 1/hy1 point by point; the volume sums are scaled once
-template <int MODE, typename T, int CBS, bool SQ = false, bool MIX = (sizeof(T) == 4)>
+// LIM (plain form): the epilogue also applies the limiter pass without TVB
+// (positivity when PP, then the divergence-free reconstruction from the
+// stage's new faces Fn) to the stage value as stored, and writes the result
+// to lout; the cells TVB changes are limited again afterwards (k_limit_list).
+// This saves the limiter pass's re-read of the stage value and its faces;
+// in the mixed form the positivity epilogue needs 254 VGPRs, which halves the
+// kernel's waves per SIMD and costs more than the pass it saves (measured)
+template <int MODE, typename T, int CBS, bool SQ = false, bool MIX = (sizeof(T) == 4), bool LIM = false,
+          bool PP = false>
 __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ FX,
                                               const T* __restrict__ FY, SRd<MIX> A,
-                                              SRd<MIX> U0, SWr<MIX> out, Geo g,
+                                              SRd<MIX> U0, SWr<MIX> out, SWr<MIX> lout,
+                                              const double* __restrict__ Fn, Geo g,
                                               const DevScal* __restrict__ sc, double dt_arg) {
   if (step_off(g)) return;
   __shared__ T sm[NPL * CBS];
@@ -192,6 +202,7 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
     }
   }
   // epilogue: mass inverse + RK combination, in double
+  double m[LIM ? NPL : 1];   // LIM: the stage value as stored (FP32 planes rounded)
   auto plane = [&](auto P) {
     const int p = P;
     const bool avg = p < 48 && (p % 6) == 0;
@@ -204,6 +215,7 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
     else if (MODE == 2) r = (3.0 / 4.0) * U0(p, c) + (1.0 / 4.0) * (A(p, c) + dt * du);
     else r = (1.0 / 3.0) * U0(p, c) + (2.0 / 3.0) * (A(p, c) + dt * du);
     out.set(p, c, r);
+    if constexpr (LIM) m[p] = (MIX && !pl_f64(p)) ? (double)(float)r : r;
   };
   // the mixed storage map needs the plane as a compile-time constant (a
   // partially unrolled loop would index acc[] dynamically, i.e. in scratch)
@@ -212,6 +224,18 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
   } else {
 #pragma unroll
     for (int p = 0; p < NPL; p++) plane(p);
+  }
+  if constexpr (LIM) {
+    // the limiter pass of k_limit<false, PP, true> on the stored stage value
+    if (interior) {
+      if (PP) pp_cell(m);
+      divfree_cell(m, Fn, g, i, j);
+#pragma unroll
+      for (int p = 0; p < NPL; p++) lout.set(p, c, m[p]);
+    } else {
+#pragma unroll
+      for (int p = 0; p < NPL; p++) lout.set(p, c, is_bplane(p) ? 0.0 : m[p]);
+    }
   }
 }
 
@@ -341,36 +365,53 @@ __global__ __launch_bounds__(BS) void k_axpy(double* __restrict__ out, const dou
 // (plain form: 128 lanes x 416 B = 53 KB per block, 3 blocks per CU)
 constexpr int CBS_F64 = 128, CBS_F32 = 256;
 
-template <int MODE>
+template <int MODE, bool LIM, bool PP>
 void launch_cell_t(mhd2d_ctx* ctx, const double* S, const double* A, const double* U0, double* out,
-                   const DevScal* sc, double dt) {
+                   double* lout, const double* Fn, const DevScal* sc, double dt) {
   const Geo& g = ctx->g;
   const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
   const long long pl = g.plane;
+  if constexpr (LIM) {   // plain form only
+    hipLaunchKernelGGL((k_cell<MODE, double, CBS_F64, false, false, true, PP>),
+                       dim3((unsigned)((n + CBS_F64 - 1) / CBS_F64)), dim3(CBS_F64), 0, ctx->stream, SRd<false>{S, pl},
+                       (const double*)ctx->FX, (const double*)ctx->FY, SRd<false>{A, pl}, SRd<false>{U0, pl},
+                       SWr<false>{out, pl}, SWr<false>{lout, pl}, Fn, g, sc, dt);
+    return;
+  }
   if (ctx->cfg.precision == MHD2D_MIXED && g.hx1 == g.hy1)
-    hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, true>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
+    hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, true, true, LIM, PP>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
                        dim3(CBS_F32), 0, ctx->stream, SRd<true>{S, pl}, (const float*)ctx->FX, (const float*)ctx->FY,
-                       SRd<true>{A, pl}, SRd<true>{U0, pl}, SWr<true>{out, pl}, g, sc, dt);
+                       SRd<true>{A, pl}, SRd<true>{U0, pl}, SWr<true>{out, pl}, SWr<true>{lout, pl}, Fn, g, sc, dt);
   else if (ctx->cfg.precision == MHD2D_MIXED)
-    hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, false>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
+    hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, false, true, LIM, PP>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
                        dim3(CBS_F32), 0, ctx->stream, SRd<true>{S, pl}, (const float*)ctx->FX, (const float*)ctx->FY,
-                       SRd<true>{A, pl}, SRd<true>{U0, pl}, SWr<true>{out, pl}, g, sc, dt);
+                       SRd<true>{A, pl}, SRd<true>{U0, pl}, SWr<true>{out, pl}, SWr<true>{lout, pl}, Fn, g, sc, dt);
   else
-    hipLaunchKernelGGL((k_cell<MODE, double, CBS_F64>), dim3((unsigned)((n + CBS_F64 - 1) / CBS_F64)),
+    hipLaunchKernelGGL((k_cell<MODE, double, CBS_F64, false, false, LIM, PP>), dim3((unsigned)((n + CBS_F64 - 1) / CBS_F64)),
                        dim3(CBS_F64), 0, ctx->stream, SRd<false>{S, pl}, (const double*)ctx->FX,
-                       (const double*)ctx->FY, SRd<false>{A, pl}, SRd<false>{U0, pl}, SWr<false>{out, pl}, g, sc,
-                       dt);
+                       (const double*)ctx->FY, SRd<false>{A, pl}, SRd<false>{U0, pl}, SWr<false>{out, pl},
+                       SWr<false>{lout, pl}, Fn, g, sc, dt);
+}
+
+template <bool LIM, bool PP>
+void launch_cell_mode(mhd2d_ctx* ctx, int mode, const double* S, const double* A, const double* U0, double* out,
+                      double* lout, const double* Fn, const DevScal* sc, double dt) {
+  switch (mode) {
+    case 0: if constexpr (!LIM) launch_cell_t<0, false, false>(ctx, S, A, U0, out, lout, Fn, sc, dt); break;
+    case 1: launch_cell_t<1, LIM, PP>(ctx, S, A, U0, out, lout, Fn, sc, dt); break;
+    case 2: launch_cell_t<2, LIM, PP>(ctx, S, A, U0, out, lout, Fn, sc, dt); break;
+    default: launch_cell_t<3, LIM, PP>(ctx, S, A, U0, out, lout, Fn, sc, dt); break;
+  }
 }
 
 int launch_cell(mhd2d_ctx* ctx, int mode, const double* S, const double* A, const double* U0, double* out,
-                const DevScal* sc, double dt) {
+                const DevScal* sc, double dt, double* lout, const double* Fn, bool pp) {
+  if (lout && (mode == 0 || ctx->cfg.precision != MHD2D_FP64))
+    return fail(ctx, -1, "launch_cell: the fused limiter is for the plain form's RK stages");
   Timed tm(ctx, KC_CELL);
-  switch (mode) {
-    case 0: launch_cell_t<0>(ctx, S, A, U0, out, sc, dt); break;
-    case 1: launch_cell_t<1>(ctx, S, A, U0, out, sc, dt); break;
-    case 2: launch_cell_t<2>(ctx, S, A, U0, out, sc, dt); break;
-    default: launch_cell_t<3>(ctx, S, A, U0, out, sc, dt); break;
-  }
+  if (!lout) launch_cell_mode<false, false>(ctx, mode, S, A, U0, out, nullptr, nullptr, sc, dt);
+  else if (pp) launch_cell_mode<true, true>(ctx, mode, S, A, U0, out, lout, Fn, sc, dt);
+  else launch_cell_mode<true, false>(ctx, mode, S, A, U0, out, lout, Fn, sc, dt);
   LAUNCHCHK(ctx);
   return 0;
 }

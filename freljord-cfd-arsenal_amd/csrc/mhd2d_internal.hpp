@@ -261,8 +261,11 @@ int launch_fluxes(mhd2d_ctx* ctx, const double* U, bool with_vertex, const DevSc
 int launch_xface_mixed(mhd2d_ctx* ctx, const double* U, int ft);                            // mhd2d_faces_mix.hip
 int launch_yface_mixed(mhd2d_ctx* ctx, const double* U, int ft);                            // mhd2d_faces_mix.hip
 int launch_vertex(mhd2d_ctx* ctx, const double* U, const DevScal* sc);                      // mhd2d_faces.hip
+// lout != nullptr: the epilogue also writes the stage value limited without
+// TVB (positivity when pp, divergence-free reconstruction from faces Fn)
 int launch_cell(mhd2d_ctx* ctx, int mode, const double* S, const double* A, const double* U0, double* out,
-                const DevScal* sc, double dt);                                                // mhd2d_cell.hip
+                const DevScal* sc, double dt, double* lout = nullptr, const double* Fn = nullptr,
+                bool pp = false);                                                             // mhd2d_cell.hip
 int launch_ct(mhd2d_ctx* ctx, int mode, const double* F, const double* A, const double* F0, double* out,
               const DevScal* sc, double dt);                                                  // mhd2d_cell.hip
 int launch_axpy(mhd2d_ctx* ctx, double* out, const double* X, double a, const double* Y, double b,
@@ -279,5 +282,9 @@ int launch_candidates_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc);  // KX
 // stage 1, and reuses the screen for stages 2 and 3); 0 for a standalone call
 int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool tvb, bool pp, bool div,
                  DevScal* sc, int stage = 0);                                                                // mhd2d_limit.hip
+// after a cell kernel with the fused limiter: detection and TVB on the
+// unlimited stage value `in`, then the changed cells limited again into out
+int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, DevScal* sc,
+                       int stage);                                                                           // mhd2d_limit.hip
 
 }  // namespace mhd2d_impl

@@ -3,172 +3,20 @@
 // and the KXRCF troubled-cell detector.
 #include "mhd2d_internal.hpp"
 #include "mhd2d_detect.hpp"
+#include "mhd2d_limit_cell.hpp"
 
 namespace mhd2d_impl {
 namespace {
 
-// ---------------------------------------------------------------------------
-// positivity limiter of one cell (pp_Limiter.f90:10-81) on modes m[]
-__device__ __forceinline__ double glleval(const double* m, int n, int i1, int j1, int fam) {
-  const double x = fam == 0 ? lamL(i1) : lam(i1);
-  const double y = fam == 0 ? lam(j1) : lamL(j1);
-  double a = 0.0;
-#pragma unroll
-  for (int b = 0; b < 10; b++) {
-    const int p = PL(b, n);
-    if (p >= 0) a += m[p] * phi(b, x, y);
-  }
-  return a;
-}
 
-__device__ __forceinline__ void pp_cell(double m[NPL]) {
-  const double eps = 1e-13;
-  // density: eta = the LAST |(rbar - eps)/(rbar - rq)| below 1.  For
-  // doubles, RN(|a/b|) < 1 exactly when |a| < |b|, so only that point's
-  // quotient is formed.
-  double eta = 1.0;
-  const double rbar = m[PL(0, 0)];
-  const double na = rbar - eps;
-  double nb = 0.0;
-  bool any = false;
-#pragma unroll
-  for (int i1 = 0; i1 < 5; i1++)
-#pragma unroll
-    for (int j1 = 0; j1 < 5; j1++)
-#pragma unroll
-      for (int f = 0; f < 2; f++) {
-        const double b = rbar - glleval(m, 0, i1, j1, f);
-        if (fabs(na) < fabs(b)) { nb = b; any = true; }
-      }
-  if (any) eta = fabs(na / nb);
-  if (eta < 1.0) {
-#pragma unroll
-    for (int b = 1; b < 6; b++) m[PL(b, 0)] = 0.5 * eta * m[PL(b, 0)];
-  }
-  // pressure.  The reference keeps eta1 from the last negative-pressure
-  // point and takes eta = min(eta, eta1) at every point, i.e. eta is the
-  // minimum of eta1 over the negative-pressure points.  With the nodes the
-  // reference holds (-1, 0, 0, 0, 1) only 30 of its 50 points are distinct;
-  // their pressures are evaluated unrolled and the bisection
-  // (pp_Limiter.f90:101-125, at most 47 halvings) runs only where p < 0.
-  double ubar[8];
-#pragma unroll
-  for (int n = 0; n < 8; n++) ubar[n] = m[PL(0, n)];
-  unsigned neg = 0u;   // bit k: distinct point k has negative pressure
-#pragma unroll
-  for (int k = 0; k < 30; k++) {
-    // k < 15: family 1 (x = -1, 0, 1; y = Gauss); k >= 15: family 2
-    const int f = k / 15, a = (k % 15) / 5, b = k % 5;
-    const double x = f == 0 ? (a == 0 ? -1.0 : a == 1 ? 0.0 : 1.0) : lam(b);
-    const double y = f == 0 ? lam(b) : (a == 0 ? -1.0 : a == 1 ? 0.0 : 1.0);
-    double uq[8];
-#pragma unroll
-    for (int n = 0; n < 8; n++) {
-      double acc = 0.0;
-#pragma unroll
-      for (int bb = 0; bb < 10; bb++) {
-        const int p = PL(bb, n);
-        if (p >= 0) acc += m[p] * phi(bb, x, y);
-      }
-      uq[n] = acc;
-    }
-    if (pressure<double>(uq) < 0.0) neg |= 1u << k;
-  }
-  eta = 1.0;
-  while (neg) {
-    const int k = __builtin_ctz(neg);
-    neg &= neg - 1u;
-    const int f = k / 15, a = (k % 15) / 5, b = k % 5;
-    const double g = a == 0 ? -1.0 : a == 1 ? 0.0 : 1.0;
-    const double lb = b == 0 ? lam(0) : b == 1 ? lam(1) : b == 2 ? lam(2) : b == 3 ? lam(3) : lam(4);
-    const double x = f == 0 ? g : lb, y = f == 0 ? lb : g;
-    double uq[8];
-    {
-      MHD_NOCONTRACT
-      const double pv[10] = {1.0, x, y, x * x - 1.0 / 3.0, x * y, y * y - 1.0 / 3.0, x * x * x - 3.0 * x / 5.0,
-                             (x * x - 1.0 / 3.0) * (y), (x) * (y * y - 1.0 / 3.0), y * y * y - 3.0 * y / 5.0};
-#pragma unroll
-      for (int n = 0; n < 8; n++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int bb = 0; bb < 10; bb++) {
-          const int p = PL(bb, n);
-          if (p >= 0) acc = fma(m[p], pv[bb], acc);
-        }
-        uq[n] = acc;
-      }
-    }
-    double ta = 0.0, tb = 1.0, ut[8];
-    while (tb - ta > 1e-14) {
-      const double tq = 0.5 * (ta + tb);
-#pragma unroll
-      for (int n = 0; n < 8; n++) ut[n] = tq * uq[n] + (1 - tq) * ubar[n];
-      if (pressure<double>(ut) < 1e-12) tb = tq;
-      else ta = tq;
-    }
-    const double tq = ta;
-    double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-    for (int n = 0; n < 8; n++) {
-      const double sq = tq * uq[n] + (1 - tq) * ubar[n];
-      s1 += (sq - ubar[n]) * (sq - ubar[n]);
-      s2 += (uq[n] - ubar[n]) * (uq[n] - ubar[n]);
-    }
-    const double eta1 = sqrt(s1) / sqrt(s2);
-    if (eta1 < eta) eta = eta1;
-  }
-  if (eta < 1.0) eta = 0.5 * eta;
-#pragma unroll
-  for (int n = 0; n < 8; n++)
-#pragma unroll
-    for (int b = 1; b < 10; b++) {
-      const int p = PL(b, n);
-      if (p >= 0) m[p] = eta * m[p];
-    }
-}
 
-// div_free_Balsara.f90:15-78: in-cell B from the four faces
-__device__ __forceinline__ void divfree_cell(double m[NPL], const double* __restrict__ F, const Geo& g, int i, int j) {
-  const long long pl = g.plane;
-  const long long cR = cix(g, i, j), cL = cix(g, i - 1, j), cD = cix(g, i, j - 1);
-  const double a0R = F[0 * pl + cR], a1R = F[1 * pl + cR], a2R = F[2 * pl + cR];
-  const double a0L = F[0 * pl + cL], a1L = F[1 * pl + cL], a2L = F[2 * pl + cL];
-  const double b0U = F[3 * pl + cR], b1U = F[4 * pl + cR], b2U = F[5 * pl + cR];
-  const double b0D = F[3 * pl + cD], b1D = F[4 * pl + cD], b2D = F[5 * pl + cD];
-  const double rxy = g.hx / g.hy, ryx = g.hy / g.hx;
-  m[PL(0, 5)] = 0.5 * (a0R + a0L) + (1.0 / 6.0) * rxy * (b1U - b1D);
-  m[PL(1, 5)] = 0.5 * (a0R - a0L) + (1.0 / 15.0) * rxy * (b2U - b2D);
-  m[PL(2, 5)] = 0.5 * (a1R + a1L);
-  m[PL(3, 5)] = -(1.0 / 4.0) * rxy * (b1U - b1D);
-  m[PL(4, 5)] = 0.5 * (a1R - a1L);
-  m[PL(5, 5)] = 0.5 * (a2R + a2L);
-  m[PL(6, 5)] = -(1.0 / 6.0) * rxy * (b2U - b2D);
-  m[PL(8, 5)] = 0.5 * (a2R - a2L);
-  m[PL(0, 6)] = 0.5 * (b0U + b0D) + (1.0 / 6.0) * ryx * (a1R - a1L);
-  m[PL(2, 6)] = 0.5 * (b0U - b0D) + (1.0 / 15.0) * ryx * (a2R - a2L);
-  m[PL(1, 6)] = 0.5 * (b1U + b1D);
-  m[PL(5, 6)] = -(1.0 / 4.0) * ryx * (a1R - a1L);
-  m[PL(4, 6)] = 0.5 * (b1U - b1D);
-  m[PL(3, 6)] = 0.5 * (b2U + b2D);
-  m[PL(9, 6)] = -(1.0 / 6.0) * ryx * (a2R - a2L);
-  m[PL(7, 6)] = 0.5 * (b2U - b2D);
-}
-
-__device__ __forceinline__ bool is_bplane(int p) { return (p >= 30 && p < 42) || p >= 48; }
-
-// limiter pass (TVB_Limiter_P2.f90:16-121 + pp + div_free_Balsara) over
-// the whole array: interior cells are limited, halo cells copied (or, with
-// DIV, their B planes zeroed as div_free_Balsara.f90:13 does).
-// flags: indicator == 2/3 reads trouble[] (1 = limit).
+// limiter of one cell (TVB_Limiter_P2.f90:16-121 + pp + div_free_Balsara):
+// interior cells are limited, halo cells copied (or, with DIV, their B
+// planes zeroed as div_free_Balsara.f90:13 does)
 template <bool APPLY, bool PP, bool DIV, bool MIX>
-__global__ __launch_bounds__(BS) void k_limit(SRd<MIX> in, SWr<MIX> out,
-                                              const double* __restrict__ F, const int* __restrict__ tmask,
-                                              const double* __restrict__ trec, Geo g) {
-  if (step_off(g)) return;
-  const long long t = xcd_block() * BS + threadIdx.x;
-  const int W = g.Nx + 2;
-  if (t >= (long long)W * (g.Ny + 2)) return;
-  const int i = (int)(t % W), j = (int)(t / W);
+__device__ __forceinline__ void limit_cell(const SRd<MIX>& in, const SWr<MIX>& out, const double* __restrict__ F,
+                                           const int* __restrict__ tmask, const double* __restrict__ trec,
+                                           const Geo& g, int i, int j) {
   const long long c = cix(g, i, j);
   const bool interior = i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny;
   if (!interior) {
@@ -208,6 +56,35 @@ __global__ __launch_bounds__(BS) void k_limit(SRd<MIX> in, SWr<MIX> out,
   for (int p = 0; p < NPL; p++) out.set(p, c, m[p]);
 }
 
+// limiter pass over the whole array
+template <bool APPLY, bool PP, bool DIV, bool MIX>
+__global__ __launch_bounds__(BS) void k_limit(SRd<MIX> in, SWr<MIX> out,
+                                              const double* __restrict__ F, const int* __restrict__ tmask,
+                                              const double* __restrict__ trec, Geo g) {
+  if (step_off(g)) return;
+  const long long t = xcd_block() * BS + threadIdx.x;
+  const int W = g.Nx + 2;
+  if (t >= (long long)W * (g.Ny + 2)) return;
+  limit_cell<APPLY, PP, DIV, MIX>(in, out, F, tmask, trec, g, (int)(t % W), (int)(t / W));
+}
+
+// the cells of the flagged list whose TVB changed something: the cell
+// kernel's fused epilogue limited every cell without TVB (pp and the
+// divergence-free reconstruction), so these are limited again from the
+// unlimited stage value, with TVB (a fixed grid strides the list)
+template <bool PP, bool MIX>
+__global__ __launch_bounds__(BS) void k_limit_list(SRd<MIX> in, SWr<MIX> out, const double* __restrict__ F,
+                                                   const int* __restrict__ list, const int* __restrict__ tmask,
+                                                   const double* __restrict__ trec, Geo g,
+                                                   const DevScal* __restrict__ lst) {
+  if (step_off(g)) return;
+  const unsigned int n = lst->nlist;
+  for (unsigned int k = blockIdx.x * BS + threadIdx.x; k < n; k += gridDim.x * BS) {
+    const int tc = list[k];
+    if (!tmask[tc]) continue;
+    limit_cell<true, PP, true, MIX>(in, out, F, tmask, trec, g, 1 + tc % g.Nx, 1 + tc / g.Nx);
+  }
+}
 
 template <bool MIX>
 __global__ __launch_bounds__(BS) void k_kxrcf(SRd<MIX> U, int* __restrict__ trouble, Geo g,
@@ -457,6 +334,38 @@ int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F,
     else if (div) LIM(false, false, true);
     else LIM(false, false, false);
 #undef LIM
+    LAUNCHCHK(ctx);
+    return 0;
+  });
+}
+
+int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, DevScal* sc,
+                       int stage) {
+  const Geo& g = ctx->g;
+  const int ind = ctx->cfg.indicator;
+  if (ind == MHD2D_IND_NONE) return 0;
+  if (ind == MHD2D_IND_ALL) return fail(ctx, -1, "launch_limit_fixup: TVB on every cell has no fused limiter");
+  Timed tm(ctx, KC_LIMIT);
+  int rc;
+  if (ind == MHD2D_IND_MLP) {
+    if (stage <= 1 && (rc = launch_mlp_screen(ctx, in))) return rc;
+    if ((rc = launch_candidates_tvb(ctx, in, sc))) return rc;
+  } else if ((rc = launch_detect_tvb(ctx, in, sc))) {
+    return rc;
+  }
+  const unsigned nb = std::min<unsigned>(nblk((long long)g.Nx * g.Ny), 1024u);
+  return with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    const SRd<MIX> vi{in, g.plane};
+    const SWr<MIX> vo{out, g.plane};
+    if (pp)
+      hipLaunchKernelGGL((k_limit_list<true, MIX>), dim3(nb), dim3(BS), 0, ctx->stream, vi, vo, F,
+                         (const int*)ctx->tlist, (const int*)ctx->tmask, (const double*)ctx->trec, g,
+                         (const DevScal*)ctx->sc);
+    else
+      hipLaunchKernelGGL((k_limit_list<false, MIX>), dim3(nb), dim3(BS), 0, ctx->stream, vi, vo, F,
+                         (const int*)ctx->tlist, (const int*)ctx->tmask, (const double*)ctx->trec, g,
+                         (const DevScal*)ctx->sc);
     LAUNCHCHK(ctx);
     return 0;
   });

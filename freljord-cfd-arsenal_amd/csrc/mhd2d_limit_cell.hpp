@@ -1,0 +1,158 @@
+// mhd2d_limit_cell.hpp -- the per-cell parts of the limiter pass that both
+// the limiter kernels and the cell kernel's fused epilogue apply: the
+// positivity limiter and the Balsara divergence-free reconstruction.
+#pragma once
+#include "mhd2d_internal.hpp"
+
+namespace mhd2d_impl {
+
+// ---------------------------------------------------------------------------
+// positivity limiter of one cell (pp_Limiter.f90:10-81) on modes m[]
+__device__ __forceinline__ double glleval(const double* m, int n, int i1, int j1, int fam) {
+  const double x = fam == 0 ? lamL(i1) : lam(i1);
+  const double y = fam == 0 ? lam(j1) : lamL(j1);
+  double a = 0.0;
+#pragma unroll
+  for (int b = 0; b < 10; b++) {
+    const int p = PL(b, n);
+    if (p >= 0) a += m[p] * phi(b, x, y);
+  }
+  return a;
+}
+
+__device__ __forceinline__ void pp_cell(double m[NPL]) {
+  const double eps = 1e-13;
+  // density: eta = the LAST |(rbar - eps)/(rbar - rq)| below 1.  For
+  // doubles, RN(|a/b|) < 1 exactly when |a| < |b|, so only that point's
+  // quotient is formed.
+  double eta = 1.0;
+  const double rbar = m[PL(0, 0)];
+  const double na = rbar - eps;
+  double nb = 0.0;
+  bool any = false;
+#pragma unroll
+  for (int i1 = 0; i1 < 5; i1++)
+#pragma unroll
+    for (int j1 = 0; j1 < 5; j1++)
+#pragma unroll
+      for (int f = 0; f < 2; f++) {
+        const double b = rbar - glleval(m, 0, i1, j1, f);
+        if (fabs(na) < fabs(b)) { nb = b; any = true; }
+      }
+  if (any) eta = fabs(na / nb);
+  if (eta < 1.0) {
+#pragma unroll
+    for (int b = 1; b < 6; b++) m[PL(b, 0)] = 0.5 * eta * m[PL(b, 0)];
+  }
+  // pressure.  The reference keeps eta1 from the last negative-pressure
+  // point and takes eta = min(eta, eta1) at every point, i.e. eta is the
+  // minimum of eta1 over the negative-pressure points.  With the nodes the
+  // reference holds (-1, 0, 0, 0, 1) only 30 of its 50 points are distinct;
+  // their pressures are evaluated unrolled and the bisection
+  // (pp_Limiter.f90:101-125, at most 47 halvings) runs only where p < 0.
+  double ubar[8];
+#pragma unroll
+  for (int n = 0; n < 8; n++) ubar[n] = m[PL(0, n)];
+  unsigned neg = 0u;   // bit k: distinct point k has negative pressure
+#pragma unroll
+  for (int k = 0; k < 30; k++) {
+    // k < 15: family 1 (x = -1, 0, 1; y = Gauss); k >= 15: family 2
+    const int f = k / 15, a = (k % 15) / 5, b = k % 5;
+    const double x = f == 0 ? (a == 0 ? -1.0 : a == 1 ? 0.0 : 1.0) : lam(b);
+    const double y = f == 0 ? lam(b) : (a == 0 ? -1.0 : a == 1 ? 0.0 : 1.0);
+    double uq[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int bb = 0; bb < 10; bb++) {
+        const int p = PL(bb, n);
+        if (p >= 0) acc += m[p] * phi(bb, x, y);
+      }
+      uq[n] = acc;
+    }
+    if (pressure<double>(uq) < 0.0) neg |= 1u << k;
+  }
+  eta = 1.0;
+  while (neg) {
+    const int k = __builtin_ctz(neg);
+    neg &= neg - 1u;
+    const int f = k / 15, a = (k % 15) / 5, b = k % 5;
+    const double g = a == 0 ? -1.0 : a == 1 ? 0.0 : 1.0;
+    const double lb = b == 0 ? lam(0) : b == 1 ? lam(1) : b == 2 ? lam(2) : b == 3 ? lam(3) : lam(4);
+    const double x = f == 0 ? g : lb, y = f == 0 ? lb : g;
+    double uq[8];
+    {
+      MHD_NOCONTRACT
+      const double pv[10] = {1.0, x, y, x * x - 1.0 / 3.0, x * y, y * y - 1.0 / 3.0, x * x * x - 3.0 * x / 5.0,
+                             (x * x - 1.0 / 3.0) * (y), (x) * (y * y - 1.0 / 3.0), y * y * y - 3.0 * y / 5.0};
+#pragma unroll
+      for (int n = 0; n < 8; n++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int bb = 0; bb < 10; bb++) {
+          const int p = PL(bb, n);
+          if (p >= 0) acc = fma(m[p], pv[bb], acc);
+        }
+        uq[n] = acc;
+      }
+    }
+    double ta = 0.0, tb = 1.0, ut[8];
+    while (tb - ta > 1e-14) {
+      const double tq = 0.5 * (ta + tb);
+#pragma unroll
+      for (int n = 0; n < 8; n++) ut[n] = tq * uq[n] + (1 - tq) * ubar[n];
+      if (pressure<double>(ut) < 1e-12) tb = tq;
+      else ta = tq;
+    }
+    const double tq = ta;
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      const double sq = tq * uq[n] + (1 - tq) * ubar[n];
+      s1 += (sq - ubar[n]) * (sq - ubar[n]);
+      s2 += (uq[n] - ubar[n]) * (uq[n] - ubar[n]);
+    }
+    const double eta1 = sqrt(s1) / sqrt(s2);
+    if (eta1 < eta) eta = eta1;
+  }
+  if (eta < 1.0) eta = 0.5 * eta;
+#pragma unroll
+  for (int n = 0; n < 8; n++)
+#pragma unroll
+    for (int b = 1; b < 10; b++) {
+      const int p = PL(b, n);
+      if (p >= 0) m[p] = eta * m[p];
+    }
+}
+
+// div_free_Balsara.f90:15-78: in-cell B from the four faces
+__device__ __forceinline__ void divfree_cell(double m[NPL], const double* __restrict__ F, const Geo& g, int i, int j) {
+  const long long pl = g.plane;
+  const long long cR = cix(g, i, j), cL = cix(g, i - 1, j), cD = cix(g, i, j - 1);
+  const double a0R = F[0 * pl + cR], a1R = F[1 * pl + cR], a2R = F[2 * pl + cR];
+  const double a0L = F[0 * pl + cL], a1L = F[1 * pl + cL], a2L = F[2 * pl + cL];
+  const double b0U = F[3 * pl + cR], b1U = F[4 * pl + cR], b2U = F[5 * pl + cR];
+  const double b0D = F[3 * pl + cD], b1D = F[4 * pl + cD], b2D = F[5 * pl + cD];
+  const double rxy = g.hx / g.hy, ryx = g.hy / g.hx;
+  m[PL(0, 5)] = 0.5 * (a0R + a0L) + (1.0 / 6.0) * rxy * (b1U - b1D);
+  m[PL(1, 5)] = 0.5 * (a0R - a0L) + (1.0 / 15.0) * rxy * (b2U - b2D);
+  m[PL(2, 5)] = 0.5 * (a1R + a1L);
+  m[PL(3, 5)] = -(1.0 / 4.0) * rxy * (b1U - b1D);
+  m[PL(4, 5)] = 0.5 * (a1R - a1L);
+  m[PL(5, 5)] = 0.5 * (a2R + a2L);
+  m[PL(6, 5)] = -(1.0 / 6.0) * rxy * (b2U - b2D);
+  m[PL(8, 5)] = 0.5 * (a2R - a2L);
+  m[PL(0, 6)] = 0.5 * (b0U + b0D) + (1.0 / 6.0) * ryx * (a1R - a1L);
+  m[PL(2, 6)] = 0.5 * (b0U - b0D) + (1.0 / 15.0) * ryx * (a2R - a2L);
+  m[PL(1, 6)] = 0.5 * (b1U + b1D);
+  m[PL(5, 6)] = -(1.0 / 4.0) * ryx * (a1R - a1L);
+  m[PL(4, 6)] = 0.5 * (b1U - b1D);
+  m[PL(3, 6)] = 0.5 * (b2U + b2D);
+  m[PL(9, 6)] = -(1.0 / 6.0) * ryx * (a2R - a2L);
+  m[PL(7, 6)] = 0.5 * (b2U - b2D);
+}
+
+__device__ __forceinline__ bool is_bplane(int p) { return (p >= 30 && p < 42) || p >= 48; }
+
+}  // namespace mhd2d_impl
