@@ -400,19 +400,23 @@ def test_mixed_shock_rotor(pkg):
     assert err < 1e-4
 
 
-def test_odd_grid_mixed_boundaries(pkg, oracle_mod):
+@pytest.mark.parametrize("ind", ["kxrcf", "none"])
+def test_odd_grid_mixed_boundaries(pkg, oracle_mod, ind):
     """37 x 23 cells (rows not a multiple of the 16-lane padding, non-square
     cells), outflow in x and periodic in y: the plain form against the oracle
     after 10 steps, and the mixed form (its non-square cell-kernel variant and
-    the double-double CT with hx != hy) within 1e-5 of the plain form"""
+    the double-double CT with hx != hy) within 1e-5 of the plain form; with
+    KXRCF-gated TVB and with no TVB (the plain form's fused limiter pass with
+    and without the list of TVB-changed cells)"""
     O = oracle_mod
     Nx, Ny, dom, bc = 37, 23, (0.0, 2 * np.pi, 0.0, 2 * np.pi), (2, 2, 1, 1)
-    o = O.Oracle(Nx, Ny, flux_type=O.HLLD, bc=bc, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=O.IND_KXRCF, Ck=1.0)
+    oi, pi = (O.IND_KXRCF, pkg.IND_KXRCF) if ind == "kxrcf" else (O.IND_NONE, pkg.IND_NONE)
+    o = O.Oracle(Nx, Ny, flux_type=O.HLLD, bc=bc, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=oi, Ck=1.0)
     o.init_data(O.IC_OT_PERTURBED, [1e-2, 5])
     runs = {}
     for name, prec in (("fp64", pkg.FP64), ("mixed", pkg.MIXED)):
         s = pkg.Solver(Nx, Ny, flux_type=pkg.HLLD, bc=bc, domain=dom, M=50.0, CFL=0.2, pp=1,
-                       indicator=pkg.IND_KXRCF, precision=prec, Ck=1.0)
+                       indicator=pi, precision=prec, Ck=1.0)
         s.upload(o.uh, o.Bx, o.By)
         s.rk3_prologue()
         for _ in range(10):
