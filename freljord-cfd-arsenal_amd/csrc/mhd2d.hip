@@ -600,6 +600,9 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
     }
   }
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
+  for (hipEvent_t* ev : {&ctx->ev_fork, &ctx->ev_faces, &ctx->ev_join})
+    if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
   if (e != hipSuccess) { mhd2d_destroy(ctx); return -2; }
   // zero everything once (halo and padding included)
   for (auto& a : al) (void)hipMemsetAsync(*a.p, 0, a.n, ctx->stream);
@@ -618,6 +621,7 @@ int mhd2d_destroy(mhd2d_ctx* ctx) {
   if (!ctx) return 0;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   for (auto& pe : ctx->prof_ev) { (void)hipEventDestroy(pe.second.first); (void)hipEventDestroy(pe.second.second); }
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   for (auto& sg : ctx->graphs) (void)hipGraphExecDestroy(sg.exec);
@@ -630,6 +634,9 @@ int mhd2d_destroy(mhd2d_ctx* ctx) {
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+  for (hipEvent_t ev : {ctx->ev_fork, ctx->ev_faces, ctx->ev_join})
+    if (ev) (void)hipEventDestroy(ev);
   free(ctx);
   return 0;
 }
@@ -1258,17 +1265,50 @@ int step_body(mhd2d_ctx* ctx) {
   // without TVB runs in the cell kernel's epilogue (the CT goes first so the
   // new faces are there), and only the cells TVB changes are limited again.
   const bool fused = ctx->cfg.indicator != MHD2D_IND_ALL && ctx->cfg.precision == MHD2D_FP64;
+  // The vertex EMFs (FP64 VALU-bound) and then the CT update run on a second
+  // stream beside the face kernels (and, unfused, the cell kernel), forked
+  // from and joined back into the context stream by events inside each stage
+  // (graph capture records the same fork and join as parallel branches).
+  // With the per-kernel event timing on, everything stays on one stream so
+  // that each kernel is timed alone.
+  const bool two = !ctx->prof;
+  auto on2 = [&](auto&& f) -> int {
+    if (!two) return f();
+    std::swap(ctx->stream, ctx->stream2);
+    const int r = f();
+    std::swap(ctx->stream, ctx->stream2);
+    return r;
+  };
+  auto wait = [&](hipEvent_t ev, hipStream_t from, hipStream_t to) -> int {
+    if (!two) return 0;
+    HIPCHK(ctx, hipEventRecord(ev, from));
+    HIPCHK(ctx, hipStreamWaitEvent(to, ev, 0));
+    return 0;
+  };
   auto stage = [&](int k, const double* Fa, const double* Fb, double* Fo, const double* A, double* out,
                    double* lim) -> int {
     int r;
-    if ((r = launch_fluxes(ctx, ctx->S, true, sc))) return r;
+    auto ct = [&] { return launch_ct(ctx, k, k == 1 ? Fa : nullptr, k == 1 ? nullptr : Fa, Fb, Fo, sc, 0.0); };
+    auto vertex = [&] { return launch_vertex(ctx, ctx->S, sc); };
+    const hipStream_t s1 = ctx->stream, s2 = ctx->stream2;
     if (fused) {
-      if ((r = launch_ct(ctx, k, k == 1 ? Fa : nullptr, k == 1 ? nullptr : Fa, Fb, Fo, sc, 0.0))) return r;
+      // vertex EMFs beside the face kernels; the cell kernel's limiter
+      // epilogue reads the new faces, so it follows the CT
+      if ((r = wait(ctx->ev_fork, s1, s2))) return r;
+      if ((r = on2(vertex))) return r;
+      if ((r = launch_fluxes(ctx, ctx->S, false, sc))) return r;
+      if ((r = wait(ctx->ev_faces, s1, s2))) return r;
+      if ((r = on2(ct))) return r;
+      if ((r = wait(ctx->ev_join, s2, s1))) return r;
       if ((r = launch_cell(ctx, k, ctx->S, A, ctx->U0, out, sc, 0.0, lim, Fo, pp))) return r;
       return launch_limit_fixup(ctx, out, lim, Fo, pp, sc, k);
     }
+    // vertex EMFs and CT beside the memory-bound cell kernel
+    if ((r = launch_fluxes(ctx, ctx->S, false, sc))) return r;
+    if ((r = wait(ctx->ev_faces, s1, s2))) return r;
+    if ((r = on2(vertex)) || (r = on2(ct))) return r;
     if ((r = launch_cell(ctx, k, ctx->S, A, ctx->U0, out, sc, 0.0))) return r;
-    if ((r = launch_ct(ctx, k, k == 1 ? Fa : nullptr, k == 1 ? nullptr : Fa, Fb, Fo, sc, 0.0))) return r;
+    if ((r = wait(ctx->ev_join, s2, s1))) return r;
     return launch_limit(ctx, out, lim, Fo, true, pp, true, sc, k);
   };
   // stage 1 (RK3.f90:151-174): set_bc; UI = S + dt L(S); S1 = limit(UI);

@@ -148,6 +148,8 @@ struct mhd2d_ctx {
   mhd2d_config cfg;
   mhd::Geo g;
   hipStream_t stream;
+  hipStream_t stream2;   // inside mhd2d_step: the vertex EMFs and CT beside the face and cell kernels
+  hipEvent_t ev_fork, ev_faces, ev_join;
   int device;
   // state buffers (52 planes each, or the mixed-form storage of
   // mhd2d_tables.hpp); roles rotate inside mhd2d_step
