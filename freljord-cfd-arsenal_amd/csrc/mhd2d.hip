@@ -601,7 +601,7 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
   }
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
-  for (hipEvent_t* ev : {&ctx->ev_fork, &ctx->ev_faces, &ctx->ev_join})
+  for (hipEvent_t* ev : {&ctx->ev_fork, &ctx->ev_faces, &ctx->ev_join, &ctx->ev_cell, &ctx->ev_det})
     if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
   if (e != hipSuccess) { mhd2d_destroy(ctx); return -2; }
   // zero everything once (halo and padding included)
@@ -635,7 +635,7 @@ int mhd2d_destroy(mhd2d_ctx* ctx) {
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
-  for (hipEvent_t ev : {ctx->ev_fork, ctx->ev_faces, ctx->ev_join})
+  for (hipEvent_t ev : {ctx->ev_fork, ctx->ev_faces, ctx->ev_join, ctx->ev_cell, ctx->ev_det})
     if (ev) (void)hipEventDestroy(ev);
   free(ctx);
   return 0;
@@ -1307,9 +1307,22 @@ int step_body(mhd2d_ctx* ctx) {
     if ((r = launch_fluxes(ctx, ctx->S, false, sc))) return r;
     if ((r = wait(ctx->ev_faces, s1, s2))) return r;
     if ((r = on2(vertex)) || (r = on2(ct))) return r;
+    if (two) HIPCHK(ctx, hipEventRecord(ctx->ev_join, s2));       // CT done
     if ((r = launch_cell(ctx, k, ctx->S, A, ctx->U0, out, sc, 0.0))) return r;
-    if ((r = wait(ctx->ev_join, s2, s1))) return r;
-    return launch_limit(ctx, out, lim, Fo, true, pp, true, sc, k);
+    const int ind = ctx->cfg.indicator;
+    if (ind != MHD2D_IND_MLP && ind != MHD2D_IND_KXRCF) {
+      if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_join, 0));
+      return launch_limit(ctx, out, lim, Fo, true, pp, true, sc, k);
+    }
+    // detection and TVB on the second stream beside the limiter pass without
+    // TVB; then the cells TVB changed are limited again
+    if ((r = wait(ctx->ev_cell, s1, s2))) return r;
+    if ((r = on2([&] { return launch_detection(ctx, out, sc, k); }))) return r;
+    if (two) HIPCHK(ctx, hipEventRecord(ctx->ev_det, s2));
+    if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_join, 0));
+    if ((r = launch_limit(ctx, out, lim, Fo, false, pp, true, sc, k))) return r;
+    if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_det, 0));
+    return launch_limit_relist(ctx, out, lim, Fo, pp);
   };
   // stage 1 (RK3.f90:151-174): set_bc; UI = S + dt L(S); S1 = limit(UI);
   // U0 keeps S with its fresh halos

@@ -149,7 +149,7 @@ struct mhd2d_ctx {
   mhd::Geo g;
   hipStream_t stream;
   hipStream_t stream2;   // inside mhd2d_step: the vertex EMFs and CT beside the face and cell kernels
-  hipEvent_t ev_fork, ev_faces, ev_join;
+  hipEvent_t ev_fork, ev_faces, ev_join, ev_cell, ev_det;
   int device;
   // state buffers (52 planes each, or the mixed-form storage of
   // mhd2d_tables.hpp); roles rotate inside mhd2d_step
@@ -284,8 +284,12 @@ int launch_candidates_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc);  // KX
 // stage 1, and reuses the screen for stages 2 and 3); 0 for a standalone call
 int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool tvb, bool pp, bool div,
                  DevScal* sc, int stage = 0);                                                                // mhd2d_limit.hip
-// after a cell kernel with the fused limiter: detection and TVB on the
-// unlimited stage value `in`, then the changed cells limited again into out
+// after a limiter pass without TVB into out (the plain form's fused cell
+// epilogue, or k_limit<false, ..>): detection and TVB on the unlimited stage
+// value `in` (KXRCF-gated and learned indicators), then the cells TVB changed
+// limited again from `in` into out.  launch_limit_fixup = both.
+int launch_detection(mhd2d_ctx* ctx, const double* in, DevScal* sc, int stage);                    // mhd2d_limit.hip
+int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp);  // mhd2d_limit.hip
 int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, DevScal* sc,
                        int stage);                                                                           // mhd2d_limit.hip
 

@@ -339,20 +339,23 @@ int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F,
   });
 }
 
-int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, DevScal* sc,
-                       int stage) {
-  const Geo& g = ctx->g;
+int launch_detection(mhd2d_ctx* ctx, const double* in, DevScal* sc, int stage) {
   const int ind = ctx->cfg.indicator;
-  if (ind == MHD2D_IND_NONE) return 0;
-  if (ind == MHD2D_IND_ALL) return fail(ctx, -1, "launch_limit_fixup: TVB on every cell has no fused limiter");
+  if (ind != MHD2D_IND_MLP && ind != MHD2D_IND_KXRCF) return 0;
   Timed tm(ctx, KC_LIMIT);
   int rc;
   if (ind == MHD2D_IND_MLP) {
     if (stage <= 1 && (rc = launch_mlp_screen(ctx, in))) return rc;
-    if ((rc = launch_candidates_tvb(ctx, in, sc))) return rc;
-  } else if ((rc = launch_detect_tvb(ctx, in, sc))) {
-    return rc;
+    return launch_candidates_tvb(ctx, in, sc);
   }
+  return launch_detect_tvb(ctx, in, sc);
+}
+
+int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp) {
+  const int ind = ctx->cfg.indicator;
+  if (ind != MHD2D_IND_MLP && ind != MHD2D_IND_KXRCF) return 0;
+  const Geo& g = ctx->g;
+  Timed tm(ctx, KC_LIMIT);
   const unsigned nb = std::min<unsigned>(nblk((long long)g.Nx * g.Ny), 1024u);
   return with_mix(ctx, [&](auto Mc) -> int {
     constexpr bool MIX = decltype(Mc)::value;
@@ -369,6 +372,15 @@ int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const doub
     LAUNCHCHK(ctx);
     return 0;
   });
+}
+
+int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, DevScal* sc,
+                       int stage) {
+  const int ind = ctx->cfg.indicator;
+  if (ind == MHD2D_IND_NONE) return 0;
+  if (ind == MHD2D_IND_ALL) return fail(ctx, -1, "launch_limit_fixup: TVB on every cell has no fused limiter");
+  int rc = launch_detection(ctx, in, sc, stage);
+  return rc ? rc : launch_limit_relist(ctx, in, out, F, pp);
 }
 
 }  // namespace mhd2d_impl
