@@ -7,6 +7,9 @@
   tend end on the same state, t and step count as the eager loop that stops
   at tend, further passes leave the state untouched, and a later, larger tend
   continues from the right buffers
+- the mixed form's double-double CT keeps div B from accumulating
+- weights set after the capture take effect in the replays
+- the stage's second stream gives the same bits as the one-stream step
 """
 import numpy as np
 import pytest
@@ -106,3 +109,20 @@ def test_graph_sees_new_weights(pkg):
     pa, pb, pc = a.mlp_probability(), b.mlp_probability(), c.mlp_probability()
     assert np.array_equal(pa, pb)
     assert (pa > 0.5).mean() > 0.5 > (pc > 0.5).mean()
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed_nn", "mixed_kxrcf"])
+def test_two_streams_equal_one_stream(pkg, form):
+    """the stage's second stream (vertex EMFs, CT, detection) against the
+    same step run on one stream (the per-kernel timing mode): bit-identical,
+    eager and graph-replayed"""
+    prec, ind = {"fp64": (pkg.FP64, pkg.IND_KXRCF), "mixed_nn": (pkg.MIXED, pkg.IND_MLP),
+                 "mixed_kxrcf": (pkg.MIXED, pkg.IND_KXRCF)}[form]
+    a, b, c = (make(pkg, prec, ind) for _ in range(3))
+    a.profile(True)                     # one stream
+    for _ in range(7):
+        a.step()
+        b.step()
+    c.step_graph(7)
+    a.profile(False)
+    assert same(a, b) and same(a, c)
