@@ -232,6 +232,9 @@ def main():
     ap.add_argument("--error", type=int, default=1, help="report mixed-vs-FP64 L2 errors (downloads both states)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--one-stream", action="store_true",
+                    help="run every step eagerly on one stream, as the profiled steps are (for a kernel trace "
+                         "whose per-kernel durations are not shared with the overlapped second stream)")
     ap.add_argument("--counters", action="store_true",
                     help="no timing: the dispatches a rocprofv3 --pmc pass reads (a streaming kernel of known "
                          "size, then two steps of each form)")
@@ -261,6 +264,8 @@ def main():
             s.mlp_set_weights(nn[0])
         s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
         s.rk3_prologue()
+        if args.one_stream:        # every step eager on one stream (the per-kernel timing mode)
+            s.profile(True)
         s.step_graph(0)            # capture the step's HIP graphs (six buffer-role phases) ahead
         s.step_graph(warmup)
         s.sync()
