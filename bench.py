@@ -275,6 +275,7 @@ def main():
         secs = time.perf_counter() - t0
         state = s.download() if args.error else None
         diag = s.diagnostics()
+        diag["face_jump"] = s.face_jump()
         tnow = s.get_time()[0]
         prof = None
         if profile_steps:
@@ -339,7 +340,7 @@ def main():
         "config": {"workload": f"perturbed Orszag-Tang {N}x{N}, P2 DG (48 DOF/cell), HLLD, TVB M=50 gated by "
                                f"KXRCF (Ck=1), positivity, SSP-RK3, CFL 0.2", "N": N, "form": args.form,
                    "indicator": "learned MLP, KXRCF in the band" if args.form == "mixed_nn" else "KXRCF",
-                   "totaldiv": d["totaldiv"], "pmin": d["pmin"],
+                   "totaldiv": d["totaldiv"], "divB_face_jump": d["face_jump"], "pmin": d["pmin"],
                    "fp64_value": results["fp64"]["value"], "fp64_ms_per_step": results["fp64"]["ms"],
                    "speedup_vs_fp64": value / results["fp64"]["value"],
                    "rel_l2_vs_fp64": errors,

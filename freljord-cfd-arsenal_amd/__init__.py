@@ -34,7 +34,7 @@ EXPORTS = (
     "mhd2d_download", "mhd2d_init_data", "mhd2d_set_bc", "mhd2d_lh", "mhd2d_lhez",
     "mhd2d_rk_stage", "mhd2d_tvb_limiter_p2", "mhd2d_pp_limiter", "mhd2d_kxrcf_detector",
     "mhd2d_div_free_balsara", "mhd2d_calculate_dt", "mhd2d_diagnostics", "mhd2d_rk3_prologue",
-    "mhd2d_step", "mhd2d_step_graph", "mhd2d_get_time", "mhd2d_set_time", "mhd2d_debug_get",
+    "mhd2d_step", "mhd2d_step_graph", "mhd2d_face_jump", "mhd2d_get_time", "mhd2d_set_time", "mhd2d_debug_get",
     "mhd2d_profile_enable", "mhd2d_profile_get",
     "mhd2d_mlp_set_weights", "mhd2d_mlp_probability", "mhd2d_indicator_stats",
     "mhd2d_trainer_create", "mhd2d_trainer_destroy", "mhd2d_trainer_last_error", "mhd2d_trainer_collect",
@@ -110,6 +110,7 @@ def load_library(path: str = LIB_PATH):
         "mhd2d_rk3_prologue": ([P], C.c_int),
         "mhd2d_step": ([P, C.c_double, dp], C.c_int),
         "mhd2d_step_graph": ([P, C.c_double, C.c_int64, dp], C.c_int),
+        "mhd2d_face_jump": ([P, dp], C.c_int),
         "mhd2d_get_time": ([P, dp, dp, i64p], C.c_int),
         "mhd2d_set_time": ([P, C.c_double, C.c_int64], C.c_int),
         "mhd2d_debug_get": ([P, i32, dp], C.c_int),
@@ -289,6 +290,12 @@ class Solver:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         self._chk(self.L.mhd2d_diagnostics(self.h, C.byref(a), C.byref(b), C.byref(c)), "diagnostics")
         return {"umax": a.value, "totaldiv": b.value, "pmin": c.value}
+
+    def face_jump(self):
+        """largest mismatch of normal B between a face and the in-cell traces (mhd2d_face_jump)"""
+        v = C.c_double()
+        self._chk(self.L.mhd2d_face_jump(self.h, C.byref(v)), "face_jump")
+        return v.value
 
     def rk3_prologue(self):
         self._chk(self.L.mhd2d_rk3_prologue(self.h), "rk3_prologue")

@@ -137,6 +137,46 @@ __global__ void k_dt_final(DevScal* sc, double hx, double hy, int advance) {
   sc->count += 1;
 }
 
+// largest mismatch between a face's normal-B polynomial and the trace of the
+// in-cell B of the interior cells on either side, over the face's Gauss
+// points: zero up to rounding after div_free_Balsara (div_free_Balsara.f90:
+// 43-58 makes each trace equal its face polynomial); halo cells are skipped
+// (their B planes are zeroed by the limiter pass)
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_facejump(SRd<MIX> U, const double* __restrict__ F, Geo g,
+                                                 unsigned long long* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  const long long nx = (long long)(g.Nx + 1) * g.Ny, ny = (long long)g.Nx * (g.Ny + 1);
+  double mx = 0.0;
+  if (t < nx + ny) {
+    const bool xf = t < nx;
+    const long long u = xf ? t : t - nx;
+    const int i = xf ? (int)(u % (g.Nx + 1)) : 1 + (int)(u % g.Nx);
+    const int j = xf ? 1 + (int)(u / (g.Nx + 1)) : (int)(u / g.Nx);
+    const int n = xf ? 5 : 6, base = xf ? 0 : 3;
+    const long long o = cix(g, i, j), cB = xf ? cix(g, i + 1, j) : cix(g, i, j + 1);
+    const bool useA = xf ? i >= 1 : j >= 1, useB = xf ? i < g.Nx : j < g.Ny;
+    const double a0 = F[base * g.plane + o], a1 = F[(base + 1) * g.plane + o], a2 = F[(base + 2) * g.plane + o];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      const double sq = lam(q), p2 = P2(sq), p3 = P3(sq), t23 = T23(sq);
+      const double face = a0 + a1 * sq + a2 * p2;
+      double ta = 0.0, tb = 0.0;
+#pragma unroll
+      for (int b = 0; b < 10; b++) {
+        const int p = PL(b, n);
+        if (p < 0) continue;
+        ta += U(p, o) * phiTr(xf ? 0 : 2, b, sq, p2, p3, t23);
+        tb += U(p, cB) * phiTr(xf ? 1 : 3, b, sq, p2, p3, t23);
+      }
+      if (useA) mx = fmax(mx, fabs(ta - face));
+      if (useB) mx = fmax(mx, fabs(tb - face));
+    }
+  }
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(mx));
+}
+
 // a step that started at t >= tend ran none of its kernels: its state and
 // face B move to the buffers that hold them after the step (the host rotates
 // the buffer roles every step)
@@ -1204,6 +1244,28 @@ int mhd2d_diagnostics(mhd2d_ctx* ctx, double* umax, double* totaldiv, double* pm
   if (umax) *umax = h.umax;
   if (totaldiv) *totaldiv = h.totaldiv;
   if (pmin) *pmin = h.pmin;
+  return 0;
+}
+
+int mhd2d_face_jump(mhd2d_ctx* ctx, double* jump) {
+  int rc = ctx_check(ctx);
+  if (rc) return rc;
+  if (!jump) return fail(ctx, -1, "face_jump: null output");
+  const Geo& g = ctx->g;
+  const long long n = (long long)(g.Nx + 1) * g.Ny + (long long)g.Nx * (g.Ny + 1);
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(ctx->partial);
+  HIPCHK(ctx, hipMemsetAsync(d, 0, sizeof *d, ctx->stream));
+  with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_facejump<MIX>), dim3(nblk(n)), dim3(BS), 0, ctx->stream, SRd<MIX>{ctx->S, g.plane},
+                       (const double*)ctx->F, g, d);
+    return 0;
+  });
+  LAUNCHCHK(ctx);
+  unsigned long long h = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *jump = __builtin_bit_cast(double, h);
   return 0;
 }
 

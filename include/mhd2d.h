@@ -138,6 +138,11 @@ int mhd2d_div_free_balsara(mhd2d_ctx *ctx);              /* div_free_Balsara.f90
 int mhd2d_calculate_dt(mhd2d_ctx *ctx, double *dt);      /* calculate_dt.f90:5-27    */
 int mhd2d_diagnostics(mhd2d_ctx *ctx, double *umax, double *totaldiv, double *pmin);
                                    /* calculate_umax/_totaldiv/_pmin.f90          */
+/* largest |face polynomial - in-cell trace| of normal B over the Gauss points
+   of every face, both interior neighbours (no counterpart in the reference:
+   the face-jump half of the divergence report; div_free_Balsara.f90:43-58
+   makes it zero up to rounding) */
+int mhd2d_face_jump(mhd2d_ctx *ctx, double *jump);
 int mhd2d_rk3_prologue(mhd2d_ctx *ctx);                  /* RK3.f90:8-18             */
 /* one pass of the RK3.f90:135-228 loop body (dt, clip to tend, 3 stages)
    with t and dt kept on the device; t_out (may be NULL) synchronises.  A

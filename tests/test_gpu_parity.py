@@ -435,3 +435,23 @@ def test_odd_grid_mixed_boundaries(pkg, oracle_mod, ind):
     assert err < TOL and rel_l2(Bx, o.Bx) < TOL and rel_l2(By, o.By) < TOL
     assert errm < 1e-5 and rel_l2(mx, Bx) < 1e-5 and rel_l2(my, By) < 1e-5
     assert dm["totaldiv"] < 1e-12 and d["totaldiv"] < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_face_jump(pkg, prec):
+    """the face-jump half of the divergence report: after the limiter pass
+    every in-cell normal-B trace equals its face polynomial up to rounding,
+    and a face changed from outside shows up as exactly that change"""
+    P = pkg.FP64 if prec == "fp64" else pkg.MIXED
+    s = pkg.Solver(48, 48, flux_type=pkg.HLLD, M=50.0, CFL=0.2, pp=1, indicator=pkg.IND_KXRCF, precision=P, Ck=1.0)
+    s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 7])
+    s.rk3_prologue()
+    for _ in range(5):
+        s.step()
+    j = s.face_jump()
+    print(f"[face jump] {prec}: {j:.2e}")
+    assert j < 1e-13
+    uh, Bx, By = s.download()
+    Bx[10, 10, 0] += 1e-6            # face i = 10 of row 11: interior cells on both sides
+    s.upload(uh, Bx, By)
+    assert abs(s.face_jump() - 1e-6) < 1e-12
