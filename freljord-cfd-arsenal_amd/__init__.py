@@ -24,6 +24,7 @@ LF, HLL, HLLC, HLLD = 1, 2, 3, 4
 IND_NONE, IND_ALL, IND_KXRCF, IND_MLP = 0, 1, 2, 3
 FP64, MIXED = 0, 1
 IC_VORTEX, IC_OT, IC_ROTOR, IC_OT_PERTURBED, IC_RIEMANN4 = 1, 3, 4, 103, 104
+FLAG_NONFINITE, FLAG_DT_SMALL, FLAG_NEG_PRESSURE = 1, 2, 4   # MHD2D_FLAG_* (positive statuses)
 IC_BLAST, IC_CLOUD_SHOCK, IC_CLOUD_SHOCK_2, IC_MAGNETIC_LOOP = 5, 6, 61, 7
 IC_BRIO_WU, IC_BRIO_WU_Y, IC_RYU_JONES, IC_RYU_JONES_Y = 8, 81, 9, 91
 DU, DEZ1, DEZ2, EZVERTEX, TROUBLE = 0, 1, 2, 3, 4
@@ -192,6 +193,7 @@ class Solver:
         if rc != 0:
             raise MHD2DError(f"mhd2d_create failed with status {rc}")
         self.h = h
+        self.flags = 0          # the last numerical flags a call returned (cleared with the time)
 
     def close(self):
         if getattr(self, "h", None):
@@ -202,9 +204,13 @@ class Solver:
         self.close()
 
     def _chk(self, rc, what):
-        if rc != 0:
+        """negative: a HIP or usage error (raises); positive: the numerical
+        flags (FLAG_*) the time loop's scalars carry, kept in self.flags"""
+        if rc < 0:
             msg = self.L.mhd2d_last_error(self.h).decode(errors="replace")
             raise MHD2DError(f"{what}: status {rc}: {msg}")
+        if rc > 0:
+            self.flags = rc
         return rc
 
     # --- state transfer -------------------------------------------------
@@ -288,8 +294,8 @@ class Solver:
 
     def diagnostics(self):
         a, b, c = C.c_double(), C.c_double(), C.c_double()
-        self._chk(self.L.mhd2d_diagnostics(self.h, C.byref(a), C.byref(b), C.byref(c)), "diagnostics")
-        return {"umax": a.value, "totaldiv": b.value, "pmin": c.value}
+        f = self._chk(self.L.mhd2d_diagnostics(self.h, C.byref(a), C.byref(b), C.byref(c)), "diagnostics")
+        return {"umax": a.value, "totaldiv": b.value, "pmin": c.value, "flags": f}
 
     def face_jump(self):
         """largest mismatch of normal B between a face and the in-cell traces (mhd2d_face_jump)"""
@@ -298,6 +304,7 @@ class Solver:
         return v.value
 
     def rk3_prologue(self):
+        self.flags = 0
         self._chk(self.L.mhd2d_rk3_prologue(self.h), "rk3_prologue")
 
     def step(self, tend=float("inf"), sync=False):
@@ -317,6 +324,7 @@ class Solver:
         return t.value, dt.value, n.value
 
     def set_time(self, t, count=0):
+        self.flags = 0
         self._chk(self.L.mhd2d_set_time(self.h, t, count), "set_time")
 
     def sync(self):

@@ -101,6 +101,7 @@ __global__ __launch_bounds__(BS) void k_dt(SRd<MIX> U, Geo g, DevScal* sc) {
     double a1, a2;
     eigenvalueMm<double, 1>(u, a1, a2);
     const double mx = fmax(fabs(a1), fabs(a2));
+    if (!isfinite(mx)) atomicOr(&sc->health, MHD2D_FLAG_NONFINITE);
     if (mx > ax) ax = mx;      // NaN never exceeds the running max
     eigenvalueMm<double, 2>(u, a1, a2);
     const double my = fmax(fabs(a1), fabs(a2));
@@ -128,8 +129,10 @@ __global__ void k_dt_final(DevScal* sc, double hx, double hy, int advance) {
   sc->alphax = ax;
   sc->alphay = ay;
   double dt = sc->CFL / (ax / hx + ay / hy);
+  if (!isfinite(dt)) sc->health |= MHD2D_FLAG_NONFINITE;
   if (!advance) { sc->dt = dt; return; }
   sc->active = 1;
+  if (dt < 1e-7) sc->health |= MHD2D_FLAG_DT_SMALL;   // the CFL step itself collapsed
   if (sc->t + dt >= sc->tend) { dt = sc->tend - sc->t; sc->t = sc->tend; }
   else sc->t = sc->t + dt;
   if (dt < 1e-7) sc->t = sc->tend;
@@ -254,6 +257,8 @@ __global__ void k_div_final(DevScal* sc, const double* partial, long long nblock
     sc->totaldiv = sqrt(tot * hx1 * hy1);
     sc->umax = __longlong_as_double((long long)sc->umax_bits);
     sc->pmin = dunkey(sc->pmin_key);
+    if (!(isfinite(sc->totaldiv) && isfinite(sc->umax) && isfinite(sc->pmin))) sc->health |= MHD2D_FLAG_NONFINITE;
+    if (sc->pmin < 0.0) sc->health |= MHD2D_FLAG_NEG_PRESSURE;
   }
 }
 
@@ -1215,7 +1220,7 @@ int mhd2d_calculate_dt(mhd2d_ctx* ctx, double* dt) {
   HIPCHK(ctx, hipMemcpyAsync(&h, ctx->sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   if (dt) *dt = h.dt;
-  return 0;
+  return h.health;
 }
 
 static int launch_diag(mhd2d_ctx* ctx) {
@@ -1244,7 +1249,7 @@ int mhd2d_diagnostics(mhd2d_ctx* ctx, double* umax, double* totaldiv, double* pm
   if (umax) *umax = h.umax;
   if (totaldiv) *totaldiv = h.totaldiv;
   if (pmin) *pmin = h.pmin;
-  return 0;
+  return h.health;
 }
 
 int mhd2d_face_jump(mhd2d_ctx* ctx, double* jump) {
@@ -1278,6 +1283,7 @@ int mhd2d_set_time(mhd2d_ctx* ctx, double t, int64_t count) {
   h.t = t;
   h.count = count;
   h.active = 1;
+  h.health = 0;
   HIPCHK(ctx, hipMemcpyAsync(ctx->sc, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return 0;
@@ -1292,7 +1298,7 @@ int mhd2d_get_time(mhd2d_ctx* ctx, double* t, double* dt, int64_t* count) {
   if (t) *t = h.t;
   if (dt) *dt = h.dt;
   if (count) *count = h.count;
-  return 0;
+  return h.health;
 }
 
 // RK3.f90:82-95
@@ -1450,7 +1456,7 @@ int read_t(mhd2d_ctx* ctx, double* t_out) {
   HIPCHK(ctx, hipMemcpyAsync(&h, ctx->sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   *t_out = h.t;
-  return 0;
+  return h.health;
 }
 
 }  // namespace

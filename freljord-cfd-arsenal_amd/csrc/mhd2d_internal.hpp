@@ -36,7 +36,7 @@ struct DevScal {
   unsigned int nlist;   // length of the troubled-cell list of the current limiter pass
   unsigned int nband;   // length of the indicator's candidate list (cells handed to KXRCF)
   int active;
-  int pad;
+  int health;           // MHD2D_FLAG_* bits, sticky until mhd2d_set_time
 };
 
 __device__ __forceinline__ long long dkey(double x) {   // order-preserving key of a double

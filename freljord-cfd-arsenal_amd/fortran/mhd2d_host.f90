@@ -223,12 +223,20 @@ module mhd2d_common
 
 contains
 
+  ! negative status: a HIP or usage error, stop.  Positive: the numerical
+  ! flags (1 non-finite value, 2 collapsed dt, 4 negative pressure), which
+  ! the reference would not notice; reported once per new combination.
   subroutine check(rc, what)
     integer(c_int), intent(in) :: rc
     character(len=*), intent(in) :: what
-    if (rc /= 0) then
+    integer(c_int), save :: reported = 0
+    if (rc < 0) then
       write(0, '(a,a,i0,a,a)') what, ': status ', rc, ': ', trim(last_error(ctx))
       error stop 1
+    else if (rc > 0 .and. rc /= reported) then
+      write(0, '(a,a,i0,a)') what, ': numerical flags ', rc, &
+        ' (1 non-finite value, 2 dt below 1e-7, 4 negative pressure)'
+      reported = rc
     end if
   end subroutine check
 

@@ -14,13 +14,24 @@
  *
  * Every function returns 0 on success, a negative value on a HIP or usage
  * error (message in mhd2d_last_error), and is stream-ordered on the
- * context's stream; functions that return scalars synchronise.
+ * context's stream; functions that return scalars synchronise.  Those that
+ * read the time loop's device scalars (mhd2d_step and mhd2d_step_graph with
+ * t_out, mhd2d_get_time, mhd2d_diagnostics, mhd2d_calculate_dt) return the
+ * context's numerical flags instead of 0 when any is set (a positive
+ * MHD2D_FLAG_* mask, outputs filled as usual): the reference has no error
+ * path, NaNs propagate and a collapsed dt ends its run.  The flags stay set
+ * until mhd2d_set_time (and so mhd2d_rk3_prologue) clears them.
  * One host thread per context; contexts are independent.
  */
 #ifndef MHD2D_H
 #define MHD2D_H
 
 #include <stdint.h>
+
+/* numerical flags (positive returns, see above) */
+#define MHD2D_FLAG_NONFINITE 1     /* a non-finite wave speed, dt or diagnostic */
+#define MHD2D_FLAG_DT_SMALL 2      /* the CFL dt fell below 1e-7 (RK3.f90:69-71 ends the run) */
+#define MHD2D_FLAG_NEG_PRESSURE 4  /* negative pressure at a quadrature point (diagnostics) */
 
 #ifdef __cplusplus
 extern "C" {

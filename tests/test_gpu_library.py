@@ -157,3 +157,34 @@ def test_checkpoint_version2_mixed(pkg, tmp_path):
     uq, qx, qy = p.download()
     assert np.array_equal(qx, Bx) and np.array_equal(qy, By)
     assert p.get_time()[2] == 3
+
+
+def test_numerical_flags(pkg):
+    """the positive statuses: a non-finite state, a negative pressure and a
+    collapsed dt are reported (the reference carries on silently), outputs
+    are still filled, and mhd2d_set_time clears the flags"""
+    s = pkg.Solver(16, 16, flux_type=pkg.HLLD, M=50.0, CFL=0.2, pp=0, indicator=pkg.IND_KXRCF, Ck=1.0)
+    s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 3])
+    s.rk3_prologue()
+    s.step()
+    d = s.diagnostics()
+    assert d["flags"] == 0 and s.flags == 0
+    uh, Bx, By = s.download()
+    bad = uh.copy()
+    bad[5, 5, 0, 0] = np.nan                       # one cell's density
+    s.upload(bad, Bx, By)
+    t = s.step(sync=True)
+    assert s.flags & pkg.FLAG_NONFINITE and np.isfinite(t)
+    s.set_time(0.0)
+    assert s.flags == 0 and s.get_time()[2] == 0
+    neg = uh.copy()
+    neg[7, 7, 0, 4] = 0.1 * neg[7, 7, 0, 4] - 5.0  # energy below the kinetic + magnetic part
+    s.upload(neg, Bx, By)
+    d = s.diagnostics()
+    assert d["pmin"] < 0 and d["flags"] & pkg.FLAG_NEG_PRESSURE
+    s.set_time(0.0)
+    big = uh.copy()
+    big[3, 3, 0, 5] = 1e12                         # a huge Bx: the fast speed collapses the CFL step
+    s.upload(big, Bx, By)
+    s.step(sync=True)
+    assert s.flags & pkg.FLAG_DT_SMALL
