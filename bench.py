@@ -232,6 +232,9 @@ def main():
     ap.add_argument("--error", type=int, default=1, help="report mixed-vs-FP64 L2 errors (downloads both states)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--graph", action="store_true",
+                    help="time HIP-graph replays of the step (mhd2d_step_graph, one stream) instead of queued "
+                         "launches on two streams")
     ap.add_argument("--one-stream", action="store_true",
                     help="run every step eagerly on one stream, as the profiled steps are (for a kernel trace "
                          "whose per-kernel durations are not shared with the overlapped second stream)")
@@ -266,11 +269,19 @@ def main():
         s.rk3_prologue()
         if args.one_stream:        # every step eager on one stream (the per-kernel timing mode)
             s.profile(True)
-        s.step_graph(0)            # capture the step's HIP graphs (six buffer-role phases) ahead
-        s.step_graph(warmup)
+        if args.graph:
+            s.step_graph(0)        # capture the step's HIP graphs (six buffer-role phases) ahead
+            s.step_graph(warmup)
+        else:
+            for _ in range(warmup):
+                s.step()
         s.sync()
         t0 = time.perf_counter()
-        s.step_graph(steps)        # one graph launch per step, no host round trip
+        if args.graph:
+            s.step_graph(steps)    # one graph launch per step (one stream), no host round trip
+        else:
+            for _ in range(steps):
+                s.step()           # queued launches on two streams, no host round trip
         s.sync()
         secs = time.perf_counter() - t0
         state = s.download() if args.error else None
@@ -340,6 +351,8 @@ def main():
         "config": {"workload": f"perturbed Orszag-Tang {N}x{N}, P2 DG (48 DOF/cell), HLLD, TVB M=50 gated by "
                                f"KXRCF (Ck=1), positivity, SSP-RK3, CFL 0.2", "N": N, "form": args.form,
                    "indicator": "learned MLP, KXRCF in the band" if args.form == "mixed_nn" else "KXRCF",
+                   "stepping": "HIP graph replays, one stream" if args.graph else
+                               ("queued launches, one stream" if args.one_stream else "queued launches, two streams"),
                    "totaldiv": d["totaldiv"], "divB_face_jump": d["face_jump"], "pmin": d["pmin"],
                    "fp64_value": results["fp64"]["value"], "fp64_ms_per_step": results["fp64"]["ms"],
                    "speedup_vs_fp64": value / results["fp64"]["value"],

@@ -1338,8 +1338,11 @@ int step_body(mhd2d_ctx* ctx) {
   // from and joined back into the context stream by events inside each stage
   // (graph capture records the same fork and join as parallel branches).
   // With the per-kernel event timing on, everything stays on one stream so
-  // that each kernel is timed alone.
-  const bool two = !ctx->prof;
+  // that each kernel is timed alone.  Graph capture also stays on one stream:
+  // with this ROCm, a captured graph with parallel branches keeps the buffers
+  // its kernels used from being released when the context frees them (about
+  // 3 GB per context at 2048^2, measured), which a single-stream graph does not.
+  const bool two = !ctx->prof && !ctx->capturing;
   auto on2 = [&](auto&& f) -> int {
     if (!two) return f();
     std::swap(ctx->stream, ctx->stream2);
@@ -1435,7 +1438,9 @@ int capture_step(mhd2d_ctx* ctx, StepGraph& sg) {
   rot_get(ctx, sg.before);
   hipGraph_t graph = nullptr;
   HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
   const int rc = step_body(ctx);
+  ctx->capturing = false;
   const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
   rot_get(ctx, sg.after);
   rot_set(ctx, sg.before);

@@ -173,6 +173,7 @@ struct mhd2d_ctx {
   mhd2d_impl::ICPar* icp;
   long long nblk_cells;
   bool lh_done;
+  bool capturing;      // inside capture_step: the stage stays on one stream
   // live kernel timing (mhd2d_profile_*): event pairs on the context stream
   bool prof;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_ev;
