@@ -77,7 +77,7 @@ typedef struct mhd2d_ic_info {
 enum { MHD2D_DU = 0, MHD2D_DEZ1 = 1, MHD2D_DEZ2 = 2, MHD2D_EZVERTEX = 3, MHD2D_TROUBLE = 4 };
 
 typedef struct mhd2d_config {
-  int32_t Nx, Ny;                 /* com.txt:1                                */
+  int32_t Nx, Ny;                 /* com.txt:1 (each >= 2)                    */
   int32_t flux_type;              /* MHD2D_LF .. MHD2D_HLLD                   */
   int32_t bcR, bcL, bcU, bcD;     /* 1 periodic, 2 outflow (set_bc.f90)       */
   int32_t pp;                     /* pp_Limiter inside TVB (TVB_Limiter_P2.f90:123) */
@@ -161,9 +161,10 @@ int mhd2d_rk3_prologue(mhd2d_ctx *ctx);                  /* RK3.f90:8-18        
    RK3.f90:135), so passes may be queued without reading t back. */
 int mhd2d_step(mhd2d_ctx *ctx, double tend, double *t_out);
 /* nsteps passes of the same loop body, each replayed as one HIP graph
-   launch (captured on first use, once per phase of the six-step rotation of
-   the context's buffers; kept until mhd2d_destroy); same results as nsteps
-   calls of mhd2d_step, bit for bit.  Passes that start at t >= tend change
+   launch (captured on first use, on one stream, once per phase of the
+   six-step rotation of the context's buffers; kept until mhd2d_destroy);
+   same results as nsteps calls of mhd2d_step, bit for bit (mhd2d_step runs
+   the vertex EMFs, CT and detection on a second stream beside the rest).  Passes that start at t >= tend change
    nothing, so a caller advances to tend with chunks of passes and one read
    of t per chunk.  nsteps = 0 captures the graphs of all six phases ahead
    and runs nothing.  While mhd2d_profile_enable is on the passes run
