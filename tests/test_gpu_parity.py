@@ -455,3 +455,39 @@ def test_face_jump(pkg, prec):
     Bx[10, 10, 0] += 1e-6            # face i = 10 of row 11: interior cells on both sides
     s.upload(uh, Bx, By)
     assert abs(s.face_jump() - 1e-6) < 1e-12
+
+
+def test_grid_minimum(pkg):
+    """fewer than 2 cells in a direction is rejected at creation"""
+    for Nx, Ny in ((1, 1), (5, 1), (1, 4)):
+        with pytest.raises(pkg.MHD2DError):
+            pkg.Solver(Nx, Ny)
+
+
+@pytest.mark.parametrize("Nx,Ny", [(2, 2), (2, 3), (5, 2), (65, 2)])
+def test_tiny_grids(pkg, oracle_mod, Nx, Ny):
+    """grids smaller than a wave or a tile (2 x 2 cells, two rows, 65 cells
+    per row): the plain form against the oracle after 3 steps, and the mixed
+    form with the learned screen stays finite (periodic)"""
+    O = oracle_mod
+    dom = (0.0, 2 * np.pi, 0.0, 2 * np.pi)
+    o = O.Oracle(Nx, Ny, flux_type=O.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=O.IND_KXRCF, Ck=1.0)
+    o.init_data(O.IC_OT_PERTURBED, [1e-2, 9])
+    s = pkg.Solver(Nx, Ny, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=pkg.IND_KXRCF, Ck=1.0)
+    s.upload(o.uh, o.Bx, o.By)
+    o.rk3_prologue()
+    s.rk3_prologue()
+    for _ in range(3):
+        o.rk3_step(1e9)
+        s.step()
+    uh, Bx, By = s.download()
+    assert rel_l2(interior(uh), interior(o.uh)) < TOL
+    assert rel_l2(Bx, o.Bx) < TOL and rel_l2(By, o.By) < TOL
+    m = pkg.Solver(Nx, Ny, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=pkg.IND_MLP,
+                   precision=pkg.MIXED, Ck=1.0)
+    m.mlp_set_weights(pkg.load_indicator_weights())
+    m.upload(o.uh, o.Bx, o.By)
+    m.rk3_prologue()
+    m.step_graph(3)
+    d = m.diagnostics()
+    assert d["flags"] == 0 and np.isfinite(m.download()[0]).all()
