@@ -5,6 +5,7 @@
 #include "mhd2d_internal.hpp"
 #include "mhd2d_limit_cell.hpp"
 
+
 namespace mhd2d_impl {
 namespace {
 
@@ -45,7 +46,15 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
                                               const double* __restrict__ Fn, Geo g,
                                               const DevScal* __restrict__ sc, double dt_arg) {
   if (step_off(g)) return;
-  __shared__ T sm[NPL * CBS];
+  // The mixed form keeps the first NR planes of the cell in registers and the
+  // other 40 in LDS: 160 B per lane, four 256-lane blocks (16 waves) per CU
+  // where 52 planes (208 B per lane) allowed three; the stage 2 and 3 kernels
+  // stay at 126 VGPRs, so a SIMD runs four waves instead of three (1.20 ->
+  // 1.13 ms per launch at 2048^2, measured).  More planes in registers would
+  // need a fifth wave's 102-VGPR limit, which spills.
+  constexpr int NR = MIX ? 12 : 0;
+  __shared__ T sm[(NPL - NR) * CBS];
+  T rg[NR > 0 ? NR : 1];
   const int tid = threadIdx.x;
   const long long t = xcd_block() * CBS + tid;
   const int W = g.Nx + 2;
@@ -63,7 +72,10 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
   for (int p = 0; p < NPL; p++) acc[p] = T(0);
   if (interior) {
 #pragma unroll
-    for (int p = 0; p < NPL; p++) sm[p * CBS + tid] = T(S(p, c));
+    for (int p = 0; p < NPL; p++) {
+      if (p < NR) rg[p < NR ? p : 0] = T(S(p, c));
+      else sm[(p - NR) * CBS + tid] = T(S(p, c));
+    }
     const T ihx1 = T(1.0 / g.hx1), ihy1 = T(1.0 / g.hy1);
     // one quadrature row; the mixed form runs rows and columns as
     // compile-time loops (the coefficients and which of them vanish fold to
@@ -74,24 +86,25 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
       const T ty = T(y), tP2y = T(P2(y)), tP3y = T(P3(y));
       int off = tid;
       asm volatile("" : "+v"(off));   // re-read the modes from LDS for every row
-      const T* m = sm + off;
+      const T* ms = sm + off;
+      auto mode = [&](int p) -> T { return p < NR ? rg[p < NR ? p : 0] : ms[(p - NR) * CBS]; };
       // U(x, y_j) = c0 + c1 x + c2 P2(x) + c3 P3(x) per variable
       T c0[8], c1[8], c2[8], c3b = T(0);
 #pragma unroll
       for (int n = 0; n < 8; n++) {
-        const T m0 = m[PL(0, n) * CBS], m1 = m[PL(1, n) * CBS];
-        const T m2 = m[PL(2, n) * CBS], m3 = m[PL(3, n) * CBS];
-        const T m4 = m[PL(4, n) * CBS], m5 = m[PL(5, n) * CBS];
+        const T m0 = mode(PL(0, n)), m1 = mode(PL(1, n));
+        const T m2 = mode(PL(2, n)), m3 = mode(PL(3, n));
+        const T m4 = mode(PL(4, n)), m5 = mode(PL(5, n));
         c0[n] = m0 + m2 * ty + m5 * tP2y;
         c1[n] = m1 + m4 * ty;
         c2[n] = m3;
         if (n == 5) {   // Bx: phi7 = P3(xi), phi9 = xi P2(eta)
-          c1[n] += m[PL(8, 5) * CBS] * tP2y;
-          c3b = m[PL(6, 5) * CBS];
+          c1[n] += mode(PL(8, 5)) * tP2y;
+          c3b = mode(PL(6, 5));
         }
         if (n == 6) {   // By: phi8 = P2(xi) eta, phi10 = P3(eta)
-          c2[n] += m[PL(7, 6) * CBS] * ty;
-          c0[n] += m[PL(9, 6) * CBS] * tP3y;
+          c2[n] += mode(PL(7, 6)) * ty;
+          c0[n] += mode(PL(9, 6)) * tP3y;
         }
       }
       auto col = [&](auto I1) {
@@ -211,7 +224,7 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
     double r;
     if (MODE == 0) r = du;
     // the LDS copy is exact for the planes stored in the quadrature type
-    else if (MODE == 1) r = (interior && (sizeof(T) == 8 || !pl_f64(p)) ? (double)sm[p * CBS + tid] : S(p, c)) + dt * du;
+    else if (MODE == 1) r = (interior && (sizeof(T) == 8 || !pl_f64(p)) ? (double)(p < NR ? rg[p < NR ? p : 0] : sm[(p - NR) * CBS + tid]) : S(p, c)) + dt * du;
     else if (MODE == 2) r = (3.0 / 4.0) * U0(p, c) + (1.0 / 4.0) * (A(p, c) + dt * du);
     else r = (1.0 / 3.0) * U0(p, c) + (2.0 / 3.0) * (A(p, c) + dt * du);
     out.set(p, c, r);
