@@ -20,8 +20,51 @@ __device__ __forceinline__ double glleval(const double* m, int n, int i1, int j1
   return a;
 }
 
+// Bound that settles a cell without its point evaluations.  On [-1,1]^2 the
+// basis functions (1, x, y, x^2-1/3, xy, y^2-1/3, x^3-3x/5, (x^2-1/3)y,
+// x(y^2-1/3), y^3-3y/5) are at most (1, 1, 1, 2/3, 1, 2/3, 2/5, 2/3, 2/3, 2/5)
+// in magnitude, so at every point a variable lies within d_n = sum_b |m_b|
+// max|phi_b| of its average, and the point sums round to within
+// 1e-12 (|ubar_n| + d_n) of the exact value.  When the density cannot move
+// by |rbar - eps| and the pressure's lower bound over those boxes is positive
+// (by more than 1e-10 of its terms, far above their rounding), no point of
+// either pass below fires, eta stays 1 and the final scaling by 1 leaves the
+// modes as they are: returning early gives the same bits.
+__device__ __forceinline__ bool pp_settled(const double m[NPL], double eps) {
+  constexpr double pmax[10] = {1.0, 1.0, 1.0, 2.0 / 3.0, 1.0, 2.0 / 3.0, 0.4, 2.0 / 3.0, 2.0 / 3.0, 0.4};
+  double d[8];
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    double a = 0.0;
+#pragma unroll
+    for (int b = 1; b < 10; b++) {
+      const int p = PL(b, n);
+      if (p >= 0) a += fabs(m[p]) * pmax[b];
+    }
+    d[n] = a * (1.0 + 1e-12) + 1e-12 * (fabs(m[PL(0, n)]) + a);
+  }
+  const double rbar = m[PL(0, 0)];
+  const double rlo = rbar - d[0];
+  if (!(d[0] < fabs(rbar - eps)) || !(rlo > 0.0)) return false;
+  double kin = 0.0, mag = 0.0;
+#pragma unroll
+  for (int n = 1; n < 4; n++) {
+    const double a = fabs(m[PL(0, n)]) + d[n];
+    kin += a * a;
+  }
+#pragma unroll
+  for (int n = 5; n < 8; n++) {
+    const double a = fabs(m[PL(0, n)]) + d[n];
+    mag += a * a;
+  }
+  const double k = 0.5 * kin / rlo, b2 = 0.5 * mag;
+  const double plo = (m[PL(0, 4)] - d[4]) - k - b2;
+  return plo > 1e-10 * (fabs(m[PL(0, 4)]) + d[4] + k + b2);
+}
+
 __device__ __forceinline__ void pp_cell(double m[NPL]) {
   const double eps = 1e-13;
+  if (pp_settled(m, eps)) return;
   // density: eta = the LAST |(rbar - eps)/(rbar - rq)| below 1.  For
   // doubles, RN(|a/b|) < 1 exactly when |a| < |b|, so only that point's
   // quotient is formed.
