@@ -83,6 +83,49 @@ def test_pp_limiter(pkg):
     assert rel_l2(out, unpack(g["out"])) < TOL
 
 
+def test_pp_limiter_near_bounds(pkg, oracle_mod):
+    """random cells whose higher modes span 1e-4 to 2 times their averages,
+    so that some cells are far inside the positivity limits, some close to
+    them and some beyond: every cell against the oracle's pp_Limiter.  The
+    kernel settles a cell by a bound on its point values before evaluating
+    the points; a cell settled wrongly would keep modes the oracle scales by
+    at most 1/2."""
+    from conftest import ACTIVE
+    N, gam = 32, 5.0 / 3.0
+    rng = np.random.default_rng(20261203)
+    uh = np.zeros((N + 2, N + 2, 10, 8), order="F")
+    rho = rng.uniform(0.2, 2.0, (N + 2, N + 2))
+    vel = rng.uniform(-1.0, 1.0, (N + 2, N + 2, 3))
+    B = rng.uniform(-1.0, 1.0, (N + 2, N + 2, 3))
+    p = rng.uniform(0.05, 1.0, (N + 2, N + 2))
+    avg = np.zeros((N + 2, N + 2, 8))
+    avg[..., 0] = rho
+    avg[..., 1:4] = rho[..., None] * vel
+    avg[..., 4] = p / (gam - 1) + 0.5 * rho * (vel ** 2).sum(-1) + 0.5 * (B ** 2).sum(-1)
+    avg[..., 5:8] = B
+    scale = 10.0 ** rng.uniform(-4.0, 0.3, (N + 2, N + 2))
+    for d, n in ACTIVE:
+        if d == 0:
+            uh[:, :, 0, n] = avg[..., n]
+        else:
+            uh[:, :, d, n] = rng.normal(size=(N + 2, N + 2)) * scale * (np.abs(avg[..., n]) + 0.1)
+    Bx = np.zeros((N + 1, N, 3))
+    By = np.zeros((N, N + 1, 3))
+    s = pkg.Solver(N, N, flux_type=4, bc=1)
+    o = oracle_mod.Oracle(N, N, flux_type=4, bc=1)
+    s.upload(uh, Bx, By)
+    o.load(uh, Bx, By)
+    s.pp_Limiter()
+    o.pp_Limiter()
+    out = interior(s.download()[0])
+    ref = interior(o.uh)
+    changed = np.abs(ref - interior(uh)).reshape(N, N, -1).max(-1) > 0
+    err = np.abs(out - ref).reshape(N, N, -1).max(-1) / np.abs(ref).reshape(N, N, -1).max(-1)
+    print(f"[pp] {changed.sum()} of {N * N} cells limited by the oracle; worst cell difference {err.max():.2e}")
+    assert 0.1 * N * N < changed.sum() < 0.9 * N * N
+    assert err.max() < 1e-10
+
+
 def test_div_free(pkg):
     g = golden("divfree_16")
     s, uh = solver_from(pkg, g)
