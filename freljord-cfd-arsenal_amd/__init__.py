@@ -42,7 +42,7 @@ EXPORTS = (
     "mhd2d_trainer_size", "mhd2d_trainer_clear", "mhd2d_trainer_get_data", "mhd2d_trainer_set_data",
     "mhd2d_trainer_grad", "mhd2d_trainer_fit",
     "mhd2d_checkpoint_save", "mhd2d_checkpoint_load", "mhd2d_ic_defaults", "mhd2d_calculate_az",
-    "mhd2d_calculate_l2error",
+    "mhd2d_calculate_l2error", "mhd2d_calculate_ez",
 )
 MLP_NPARAM = 8129        # W1[60][64], b1[64], W2[64][64], b2[64], w3[64], b3
 MLP_NIN = 60
@@ -135,6 +135,7 @@ def load_library(path: str = LIB_PATH):
         "mhd2d_ic_defaults": ([i32, C.POINTER(ICInfo)], C.c_int),
         "mhd2d_calculate_az": ([P], C.c_int),
         "mhd2d_calculate_l2error": ([P, C.c_double, dp], C.c_int),
+        "mhd2d_calculate_ez": ([C.c_int64, dp, dp, dp, dp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -174,6 +175,18 @@ def load_indicator_weights(path=WEIGHTS_PATH):
     if p.size != MLP_NPARAM:
         raise MHD2DError(f"{path}: {p.size} parameters, expected {MLP_NPARAM}")
     return p
+
+
+def calculate_Ez(u1, u2, B1, B2):
+    """calculate_Ez.f90:1-7, elementwise: Ez = u2*B1 - u1*B2 (mhd2d_calculate_ez)"""
+    L = load_library()
+    a, b, c, d = (np.ascontiguousarray(np.broadcast_to(x, np.broadcast(u1, u2, B1, B2).shape), dtype=np.float64)
+                  for x in (u1, u2, B1, B2))
+    out = np.empty(a.shape)
+    rc = L.mhd2d_calculate_ez(a.size, _dp(a), _dp(b), _dp(c), _dp(d), _dp(out))
+    if rc != 0:
+        raise MHD2DError(f"mhd2d_calculate_ez failed with status {rc}")
+    return out
 
 
 class Solver:

@@ -13,7 +13,7 @@
 //                      fluxes, Davis bounds, 1D Riemann flux, face EMF
 //   k_vertex           one lane per vertex: 2D Riemann (corner) EMF
 //   k_cell<MODE>       one lane per cell: volume + surface integrals, mass
-//                      inverse, SSP-RK3 combination (RK3.f90:157-212)
+//                      inverse, SSP-RK3 combination (RK3.f90:80-135)
 //   k_ct<MODE>         one lane per face: CT right-hand side + combination
 //   k_limit            one lane per cell: TVB (+ positivity) + Balsara
 //                      divergence-free reconstruction
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(BS) void k_dt(SRd<MIX> U, Geo g, DevScal* sc) {
   }
 }
 
-// RK3.f90:137-148: dt from the maxima, clipped to tend; steps that start at
+// RK3.f90:60-71: dt from the maxima, clipped to tend; steps that start at
 // t >= tend are switched off (active = 0) like the reference's loop test.
 __global__ void k_dt_final(DevScal* sc, double hx, double hy, int advance) {
   if (threadIdx.x != 0) return;
@@ -305,12 +305,12 @@ __device__ void ic_eval(const ICPar& c, double x, double y, double U[8]) {
   } else if (c.ic == MHD2D_IC_BLAST) {                   // init5.txt:3-18, p0.f90:1-14
     p = sqrt(x * x + y * y) <= 0.1 ? 1000.0 : 0.1;
     B1 = 100.0 / sqrt(4 * pi);
-  } else if (c.ic == MHD2D_IC_CLOUD_SHOCK) {            // Cloud_Shock_functions.f90:1-118
+  } else if (c.ic == MHD2D_IC_CLOUD_SHOCK) {            // Cloud_Shock_functions.f90:1-117
     const double r = sqrt((x - 1.4) * (x - 1.4) + (y - 0.5) * (y - 0.5));
     if (x <= 1.2) { rho = 3.88968; v3 = -0.05234; p = 14.2641; B3 = 3.9353; }
     else { rho = r >= 0.18 ? 1.0 : 5.0; v1 = -3.3156; p = 0.04; B3 = 1.0; }
     B1 = 1.0;
-  } else if (c.ic == MHD2D_IC_CLOUD_SHOCK_2) {          // Cloud_Shock_functions2.f90:1-118
+  } else if (c.ic == MHD2D_IC_CLOUD_SHOCK_2) {          // Cloud_Shock_functions2.f90:1-116
     const double r = sqrt((x - 0.25) * (x - 0.25) + (y - 0.5) * (y - 0.5));
     if (x <= 0.05) { rho = 3.88968; v1 = 11.2536; p = 167.345; B2 = 0.21826182; B3 = -0.21826182; }
     else { rho = r >= 0.15 ? 1.0 : 10.0; p = 1.0; B2 = 0.56418958; B3 = 0.56418958; }
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(BS) void k_init_faces(double* __restrict__ F, Geo g
   }
 }
 
-// calculate_Az.f90:13-26 with Loop_functions.f90:37-52 (Az of the magnetic loop)
+// calculate_Az.f90:13-26 with Loop_functions.f90:37-51 (Az of the magnetic loop)
 __device__ double loop_az(double x, double y) {
   MHD_NOCONTRACT
   const double r = sqrt(x * x + y * y);
@@ -967,7 +967,9 @@ int mhd2d_checkpoint_save(mhd2d_ctx* ctx, const char* path) {
   h.flux_type = ctx->cfg.flux_type; h.precision = ctx->cfg.precision;
   h.bcR = g.bcR; h.bcL = g.bcL; h.bcU = g.bcU; h.bcD = g.bcD;
   h.xa = ctx->cfg.xa; h.xb = ctx->cfg.xb; h.ya = ctx->cfg.ya; h.yb = ctx->cfg.yb;
-  if ((rc = mhd2d_get_time(ctx, &h.t, &h.dt, &h.count))) return rc;
+  // a positive status only reports the sticky numerical flags: the state is
+  // still saved (a flagged run is the one most worth keeping)
+  if ((rc = mhd2d_get_time(ctx, &h.t, &h.dt, &h.count)) < 0) return rc;
   h.payload_bytes = pay.size() * sizeof(double);
   const Crc32& C = crc32_tables();
   uint32_t crc = C.update(0, &h, sizeof h);
@@ -998,6 +1000,11 @@ int mhd2d_checkpoint_load(mhd2d_ctx* ctx, const char* path) {
   else if (memcmp(h.magic, "MHD2DCK1", 8) || (h.version != 1 && h.version != 2) || h.header_bytes != sizeof h)
     why = "not a version-1 or version-2 checkpoint";
   else if (h.Nx != g.Nx || h.Ny != g.Ny || h.degree != 2 || h.nplanes != 52) why = "grid differs from the context";
+  else if (h.xa != ctx->cfg.xa || h.xb != ctx->cfg.xb || h.ya != ctx->cfg.ya || h.yb != ctx->cfg.yb)
+    why = "domain differs from the context";
+  else if (h.bcR != g.bcR || h.bcL != g.bcL || h.bcU != g.bcU || h.bcD != g.bcD)
+    why = "boundary conditions differ from the context";
+  else if (h.flux_type != ctx->cfg.flux_type) why = "flux_type differs from the context";
   else if (h.payload_bytes != (52 * W * H + nbx + nby + (h.version == 2 ? (nbx + nby) / 3 : 0)) * sizeof(double))
     why = "payload size differs";
   if (!why) {
@@ -1113,7 +1120,7 @@ int mhd2d_lhez(mhd2d_ctx* ctx) {
   return launch_ct(ctx, 0, nullptr, nullptr, nullptr, ctx->dEz, nullptr, 0.0);
 }
 
-// RK3.f90:157-168 (stage 1), :183-193 (stage 2), :206-212 (stage 3)
+// RK3.f90:80-91 (stage 1), :106-116 (stage 2), :129-135 (stage 3)
 int mhd2d_rk_stage(mhd2d_ctx* ctx, int32_t stage, double dt) {
   int rc = ctx_check(ctx);
   if (rc) return rc;
@@ -1274,6 +1281,13 @@ int mhd2d_face_jump(mhd2d_ctx* ctx, double* jump) {
   return 0;
 }
 
+int mhd2d_calculate_ez(int64_t n, const double* u1, const double* u2, const double* B1, const double* B2,
+                       double* Ez) {
+  if (n < 0 || (n > 0 && (!u1 || !u2 || !B1 || !B2 || !Ez))) return -1;
+  for (int64_t k = 0; k < n; k++) Ez[k] = calculate_ez(u1[k], u2[k], B1[k], B2[k]);
+  return 0;
+}
+
 int mhd2d_set_time(mhd2d_ctx* ctx, double t, int64_t count) {
   int rc = ctx_check(ctx);
   if (rc) return rc;
@@ -1301,7 +1315,7 @@ int mhd2d_get_time(mhd2d_ctx* ctx, double* t, double* dt, int64_t* count) {
   return h.health;
 }
 
-// RK3.f90:82-95
+// RK3.f90:5-18
 int mhd2d_rk3_prologue(mhd2d_ctx* ctx) {
   int rc;
   if ((rc = mhd2d_set_time(ctx, 0.0, 0))) return rc;
@@ -1315,7 +1329,7 @@ int mhd2d_rk3_prologue(mhd2d_ctx* ctx) {
 
 namespace {
 
-// the kernels of one pass of the RK3.f90:135-228 loop body, from dt to the
+// the kernels of one pass of the RK3.f90:58-174 loop body, from dt to the
 // new state (tend is set by the caller); rotates the buffer roles
 int step_body(mhd2d_ctx* ctx) {
   struct GateOn {
@@ -1395,17 +1409,17 @@ int step_body(mhd2d_ctx* ctx) {
     if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_det, 0));
     return launch_limit_relist(ctx, out, lim, Fo, pp);
   };
-  // stage 1 (RK3.f90:151-174): set_bc; UI = S + dt L(S); S1 = limit(UI);
+  // stage 1 (RK3.f90:74-97): set_bc; UI = S + dt L(S); S1 = limit(UI);
   // U0 keeps S with its fresh halos
   if ((rc = launch_setbc(ctx, ctx->S, sc))) return rc;
   if ((rc = stage(1, ctx->F, nullptr, ctx->FI, nullptr, ctx->UI, ctx->T))) return rc;
-  if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;          // RK3.f90:174
+  if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;          // RK3.f90:97
   std::swap(ctx->U0, ctx->S);   // U0 <- start-of-step state
   std::swap(ctx->S, ctx->T);    // S  <- limited stage-1 state
   std::swap(ctx->F0, ctx->F);   // F0 <- start-of-step faces; F is free
   // stage 2: UII = 3/4 U0 + 1/4 (UI + dt L(S)); S2 = limit(UII)
   if ((rc = stage(2, ctx->FI, ctx->F0, ctx->FII, ctx->UI, ctx->UII, ctx->T))) return rc;
-  if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;          // RK3.f90:200 (stage 3 set_bc)
+  if ((rc = launch_setbc(ctx, ctx->T, sc))) return rc;          // RK3.f90:123 (stage 3 set_bc)
   std::swap(ctx->S, ctx->T);
   // stage 3: T = 1/3 U0 + 2/3 (UII + dt L(S)); S_new = limit(T)
   if ((rc = stage(3, ctx->FII, ctx->F0, ctx->F, ctx->UII, ctx->T, ctx->UI))) return rc;
@@ -1468,7 +1482,7 @@ int read_t(mhd2d_ctx* ctx, double* t_out) {
 
 extern "C" {
 
-// one pass of the RK3.f90:135-228 loop body with t and dt on the device
+// one pass of the RK3.f90:58-174 loop body with t and dt on the device
 int mhd2d_step(mhd2d_ctx* ctx, double tend, double* t_out) {
   int rc = ctx_check(ctx);
   if (rc) return rc;

@@ -13,9 +13,9 @@ namespace {
 // cell residual (Lh.f90:11-68 volume, :268-304 surface + mass inverse) and
 // the RK3 combination of RK3.f90 as the epilogue:
 //   MODE 0: du                     (Lh alone)
-//   MODE 1: out = S + dt du        (uI,  RK3.f90:157)
-//   MODE 2: out = 3/4 U0 + 1/4 (UI + dt du)   (RK3.f90:183,187)
-//   MODE 3: out = 1/3 U0 + 2/3 (UII + dt du)  (RK3.f90:208-210)
+//   MODE 1: out = S + dt du        (uI,  RK3.f90:80)
+//   MODE 2: out = 3/4 U0 + 1/4 (UI + dt du)   (RK3.f90:106,110)
+//   MODE 3: out = 1/3 U0 + 2/3 (UII + dt du)  (RK3.f90:131,133)
 // The launch covers the halo ring too: there du = 0 (Lh.f90:5), so the
 // combination carries the start-of-step halos exactly as the reference's
 // whole-array statements do (the stale halos TVB later reads).
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
 }
 
 // constrained transport: LhEz.f90:71-96 (+ the face combinations of
-// RK3.f90:159-212 for MODE 1..3 as in k_cell).  Face planes: 0..2 Bx modes
+// RK3.f90:82-83, 107-112, 129-135 for MODE 1..3 as in k_cell).  Face planes: 0..2 Bx modes
 // on x faces (i = 0..Nx, j = 1..Ny), 3..5 By modes on y faces (i = 1..Nx,
 // j = 0..Ny).
 template <int MODE, typename T>

@@ -307,10 +307,14 @@ __device__ __forceinline__ void riemann(T SR, T SL, const T UL[8], const T UR[8]
   else hllcd_flux<T, DIR, FT == 4>(SR, SL, UL, UR, FL, FR, F, Ustar);
 }
 
-// corner electric field v*B1 - u*B2 (calculate_Ez.f90:5)
+// calculate_Ez.f90:1-7: the corner electric field Ez = u2*B1 - u1*B2
+template <typename T> __host__ __device__ __forceinline__ T calculate_ez(T u1, T u2, T B1, T B2) {
+  return u2 * B1 - u1 * B2;
+}
+// the same from a conserved state (velocities through one reciprocal)
 template <typename T> __device__ __forceinline__ T corner_ez(const T U[8]) {
   const T ir = T(1) / U[0];
-  return (U[2] * ir) * U[5] - (U[1] * ir) * U[6];
+  return calculate_ez(U[1] * ir, U[2] * ir, U[5], U[6]);
 }
 
 // vertex EMF: LF_Flux_2D.f90:13-60, HLL_Flux_2D.f90:17-109 (uninitialised
@@ -337,7 +341,7 @@ __device__ __forceinline__ T vertex_emf(const T URU[8], const T ULU[8], const T 
   const T SL = tmin(tmin(tmin(bRU, bLU), bRD), bLD);
   const T SU = tmax(tmax(tmax(cRU, cLU), cRD), cLD);
   const T SD = tmin(tmin(tmin(dRU, dLU), dRD), dLD);
-  // the selection of HLL_Flux_2D.f90:214-224 (same in HLLC/HLLD)
+  // the selection of HLL_Flux_2D.f90:97-107 (same in HLLC/HLLD)
   T FxA[8], FyA[8], FxB[8], FyB[8];
   T EzLs, EzRs, EzDs, EzUs;
   if (FT == 2) {

@@ -36,7 +36,7 @@ __host__ __device__ constexpr double lamL(int q) {
   return q == 0 ? -1.0 : q == 4 ? 1.0 : 0.0;
 }
 
-// Legendre-product basis phi_{b+1}(x, y), b = 0..9 (get_basis.f90:135-245)
+// Legendre-product basis phi_{b+1}(x, y), b = 0..9 (get_basis.f90:49-159)
 __host__ __device__ constexpr double phi(int b, double x, double y) {
   MHD_NOCONTRACT
   return b == 0 ? 1.0

@@ -5,6 +5,7 @@
 ! This file keeps that shape for a Fortran caller:
 !
 !   module mhd2d_capi    iso_c_binding interfaces of include/mhd2d.h
+!                        (mhd2d_capi.f90)
 !   module mhd2d_common  the host-side state a caller of the reference
 !                        reads: uh, Bx, By in the reference bounds, t, dt,
 !                        tend, umax, totaldiv, pmin, count, plus the
@@ -12,7 +13,9 @@
 !   external subroutines set_bc, Lh, LhEz, TVB_Limiter_P2, pp_Limiter,
 !                        KXRCF_Detector, div_free_Balsara, calculate_dt,
 !                        calculate_umax, calculate_totaldiv, calculate_pmin,
-!                        init_data, RK3, save_solution, save_XY
+!                        init_data, calculate_Ez, calculate_Az,
+!                        calculate_L2error, RK3 (with the frame files of
+!                        RK3.f90:29-53,154-172), save_solution, save_XY
 !
 ! Every subroutine body is one call into the C ABI on the context; the
 ! numerical state lives on the GPU between calls.  uh/Bx/By on the host are
@@ -21,187 +24,6 @@
 ! Errors: every status is checked; a failing call prints the library's
 ! message and stops the program (the reference has no error path).
 
-module mhd2d_capi
-  use iso_c_binding
-  implicit none
-
-  integer(c_int32_t), parameter :: MHD2D_LF = 1, MHD2D_HLL = 2, MHD2D_HLLC = 3, MHD2D_HLLD = 4
-  integer(c_int32_t), parameter :: MHD2D_IND_NONE = 0, MHD2D_IND_ALL = 1, MHD2D_IND_KXRCF = 2, &
-                                   MHD2D_IND_MLP = 3
-  integer(c_int32_t), parameter :: MHD2D_FP64 = 0, MHD2D_MIXED = 1
-  integer(c_int32_t), parameter :: MHD2D_IC_VORTEX = 1, MHD2D_IC_ORSZAG_TANG = 3, MHD2D_IC_ROTOR = 4, &
-                                   MHD2D_IC_BLAST = 5, MHD2D_IC_CLOUD_SHOCK = 6, MHD2D_IC_CLOUD_SHOCK_2 = 61, &
-                                   MHD2D_IC_MAGNETIC_LOOP = 7, MHD2D_IC_BRIO_WU = 8, MHD2D_IC_BRIO_WU_Y = 81, &
-                                   MHD2D_IC_RYU_JONES = 9, MHD2D_IC_RYU_JONES_Y = 91, &
-                                   MHD2D_IC_OT_PERTURBED = 103, MHD2D_IC_RIEMANN4 = 104
-
-  ! struct mhd2d_ic_info (include/mhd2d.h)
-  type, bind(C) :: mhd2d_ic_info
-    real(c_double) :: xa, xb, ya, yb, tend, M
-    integer(c_int32_t) :: bcR, bcL, bcU, bcD, shock, pad
-  end type mhd2d_ic_info
-
-  ! struct mhd2d_config (include/mhd2d.h)
-  type, bind(C) :: mhd2d_config
-    integer(c_int32_t) :: Nx, Ny, flux_type, bcR, bcL, bcU, bcD, pp, indicator, precision, reserved
-    real(c_double) :: xa, xb, ya, yb, M, CFL, Ck, mlp_lo, mlp_hi
-  end type mhd2d_config
-
-  interface
-    integer(c_int) function mhd2d_create(cfg, ctx) bind(C, name='mhd2d_create')
-      import :: c_int, c_ptr, mhd2d_config
-      type(mhd2d_config), intent(in) :: cfg
-      type(c_ptr), intent(out) :: ctx
-    end function
-    integer(c_int) function mhd2d_destroy(ctx) bind(C, name='mhd2d_destroy')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    type(c_ptr) function mhd2d_last_error(ctx) bind(C, name='mhd2d_last_error')
-      import :: c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    integer(c_int) function mhd2d_sync(ctx) bind(C, name='mhd2d_sync')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    integer(c_int) function mhd2d_upload(ctx, uh, Bx, By) bind(C, name='mhd2d_upload')
-      import :: c_int, c_ptr, c_double
-      type(c_ptr), value :: ctx
-      real(c_double), intent(in) :: uh(*), Bx(*), By(*)
-    end function
-    integer(c_int) function mhd2d_download(ctx, uh, Bx, By) bind(C, name='mhd2d_download')
-      import :: c_int, c_ptr, c_double
-      type(c_ptr), value :: ctx
-      real(c_double), intent(out) :: uh(*), Bx(*), By(*)
-    end function
-    integer(c_int) function mhd2d_init_data(ctx, ic, extra, nextra) bind(C, name='mhd2d_init_data')
-      import :: c_int, c_ptr, c_double, c_int32_t
-      type(c_ptr), value :: ctx
-      integer(c_int32_t), value :: ic, nextra
-      real(c_double), intent(in) :: extra(*)
-    end function
-    integer(c_int) function mhd2d_set_bc(ctx) bind(C, name='mhd2d_set_bc')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    integer(c_int) function mhd2d_lh(ctx) bind(C, name='mhd2d_lh')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    integer(c_int) function mhd2d_lhez(ctx) bind(C, name='mhd2d_lhez')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    integer(c_int) function mhd2d_rk_stage(ctx, stage, dt) bind(C, name='mhd2d_rk_stage')
-      import :: c_int, c_ptr, c_int32_t, c_double
-      type(c_ptr), value :: ctx
-      integer(c_int32_t), value :: stage
-      real(c_double), value :: dt
-    end function
-    integer(c_int) function mhd2d_tvb_limiter_p2(ctx) bind(C, name='mhd2d_tvb_limiter_p2')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    integer(c_int) function mhd2d_pp_limiter(ctx) bind(C, name='mhd2d_pp_limiter')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    integer(c_int) function mhd2d_kxrcf_detector(ctx, nflag) bind(C, name='mhd2d_kxrcf_detector')
-      import :: c_int, c_ptr, c_int64_t
-      type(c_ptr), value :: ctx
-      integer(c_int64_t), intent(out) :: nflag
-    end function
-    integer(c_int) function mhd2d_div_free_balsara(ctx) bind(C, name='mhd2d_div_free_balsara')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    integer(c_int) function mhd2d_calculate_dt(ctx, dt) bind(C, name='mhd2d_calculate_dt')
-      import :: c_int, c_ptr, c_double
-      type(c_ptr), value :: ctx
-      real(c_double), intent(out) :: dt
-    end function
-    integer(c_int) function mhd2d_diagnostics(ctx, umax, totaldiv, pmin) bind(C, name='mhd2d_diagnostics')
-      import :: c_int, c_ptr, c_double
-      type(c_ptr), value :: ctx
-      real(c_double), intent(out) :: umax, totaldiv, pmin
-    end function
-    integer(c_int) function mhd2d_rk3_prologue(ctx) bind(C, name='mhd2d_rk3_prologue')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    integer(c_int) function mhd2d_step(ctx, tend, t_out) bind(C, name='mhd2d_step')
-      import :: c_int, c_ptr, c_double
-      type(c_ptr), value :: ctx
-      real(c_double), value :: tend
-      real(c_double), intent(out) :: t_out
-    end function
-    integer(c_int) function mhd2d_step_graph(ctx, tend, nsteps, t_out) bind(C, name='mhd2d_step_graph')
-      import :: c_int, c_ptr, c_double, c_int64_t
-      type(c_ptr), value :: ctx
-      real(c_double), value :: tend
-      integer(c_int64_t), value :: nsteps
-      real(c_double), intent(out) :: t_out
-    end function
-    integer(c_int) function mhd2d_get_time(ctx, t, dt, count) bind(C, name='mhd2d_get_time')
-      import :: c_int, c_ptr, c_double, c_int64_t
-      type(c_ptr), value :: ctx
-      real(c_double), intent(out) :: t, dt
-      integer(c_int64_t), intent(out) :: count
-    end function
-    integer(c_int) function mhd2d_set_time(ctx, t, count) bind(C, name='mhd2d_set_time')
-      import :: c_int, c_ptr, c_double, c_int64_t
-      type(c_ptr), value :: ctx
-      real(c_double), value :: t
-      integer(c_int64_t), value :: count
-    end function
-    integer(c_int) function mhd2d_ic_defaults(ic, info) bind(C, name='mhd2d_ic_defaults')
-      import :: c_int, c_int32_t, mhd2d_ic_info
-      integer(c_int32_t), value :: ic
-      type(mhd2d_ic_info), intent(out) :: info
-    end function
-    integer(c_int) function mhd2d_calculate_az(ctx) bind(C, name='mhd2d_calculate_az')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    integer(c_int) function mhd2d_calculate_l2error(ctx, t20, L2) bind(C, name='mhd2d_calculate_l2error')
-      import :: c_int, c_ptr, c_double
-      type(c_ptr), value :: ctx
-      real(c_double), value :: t20
-      real(c_double), intent(out) :: L2(8)
-    end function
-    integer(c_int) function mhd2d_checkpoint_save(ctx, path) bind(C, name='mhd2d_checkpoint_save')
-      import :: c_int, c_ptr, c_char
-      type(c_ptr), value :: ctx
-      character(kind=c_char), intent(in) :: path(*)
-    end function
-    integer(c_int) function mhd2d_checkpoint_load(ctx, path) bind(C, name='mhd2d_checkpoint_load')
-      import :: c_int, c_ptr, c_char
-      type(c_ptr), value :: ctx
-      character(kind=c_char), intent(in) :: path(*)
-    end function
-  end interface
-
-contains
-
-  ! copy the NUL-terminated message of mhd2d_last_error into a Fortran string
-  function last_error(ctx) result(msg)
-    type(c_ptr), intent(in) :: ctx
-    character(len=256) :: msg
-    character(kind=c_char), pointer :: s(:)
-    type(c_ptr) :: p
-    integer :: n
-    msg = ''
-    p = mhd2d_last_error(ctx)
-    if (.not. c_associated(p)) return
-    call c_f_pointer(p, s, [256])
-    do n = 1, 256
-      if (s(n) == c_null_char) exit
-      msg(n:n) = s(n)
-    end do
-  end function last_error
-
-end module mhd2d_capi
 
 
 module mhd2d_common
@@ -220,6 +42,17 @@ module mhd2d_common
   real(c_double) :: t20 = 0, L2(8) = 0
   integer(c_int64_t) :: count = 0, nflag = 0
   logical :: verbose = .true.
+  ! the initial condition init_data projects: the reference picks it at
+  ! compile time (include 'initN.txt', init_data.f90:19); here it is set
+  ! before the call (ids and parameters of mhd2d_init_data in mhd2d.h)
+  integer :: ic_select = MHD2D_IC_ORSZAG_TANG, ic_nextra = 0
+  real(c_double) :: ic_extra(64) = 0
+  ! frame files of RK3 (com.txt:1 flash, frameMAX): flash = 1 writes
+  ! Q1..Q8flash.txt and T.txt as RK3.f90:29-53,154-172 do, flash = 0 only
+  ! the t = 0 frame (the reference opens and writes it whatever flash is),
+  ! flash = -1 (this host's default) no frame files at all
+  integer :: flash = -1, frameMAX = 200
+  integer, private :: frame_unit(9) = 0
 
 contains
 
@@ -277,6 +110,43 @@ contains
   subroutine sync_time()
     call check(mhd2d_get_time(ctx, t, dt, count), 'mhd2d_get_time')
   end subroutine sync_time
+
+  ! RK3.f90:29-37: the frame files (replaced if they exist)
+  subroutine frame_open()
+    integer :: n
+    character(len=16) :: fname
+    do n = 1, NumEq
+      write(fname, '(a,i0,a)') 'Q', n, 'flash.txt'
+      open(newunit=frame_unit(n), file=trim(fname), status='replace')
+    end do
+    open(newunit=frame_unit(9), file='T.txt', status='replace')
+  end subroutine frame_open
+
+  ! RK3.f90:39-53 / :156-170: one frame, every mode d of every interior
+  ! cell in d -> j -> i order, one value per line and file, then t.  Only a
+  ! frame downloads the state.
+  subroutine frame_write()
+    integer :: i, j, d, n
+    call gpu_download()
+    do d = 1, dimPk
+      do j = 1, Ny
+        do i = 1, Nx
+          do n = 1, NumEq
+            write(frame_unit(n), *) uh(i, j, d, n)
+          end do
+        end do
+      end do
+    end do
+    write(frame_unit(9), *) t
+  end subroutine frame_write
+
+  subroutine frame_close()
+    integer :: n
+    do n = 1, 9
+      if (frame_unit(n) /= 0) close(frame_unit(n))
+      frame_unit(n) = 0
+    end do
+  end subroutine frame_close
 
 end module mhd2d_common
 
@@ -342,13 +212,36 @@ subroutine calculate_pmin()        ! calculate_pmin.f90:5-26
   call check(mhd2d_diagnostics(ctx, a, b, pmin), 'calculate_pmin')
 end subroutine calculate_pmin
 
-! init_data.f90:21-121 on the device for the ICs of include/mhd2d.h
-subroutine init_data(ic, extra, nextra)
+! init_data.f90:1-121 on the device, argument-less like the reference: the
+! initial condition is ic_select with ic_extra(1:ic_nextra) (mhd2d_common)
+subroutine init_data()
+  use mhd2d_common
+  call check(mhd2d_init_data(ctx, int(ic_select, c_int32_t), ic_extra, int(ic_nextra, c_int32_t)), 'init_data')
+end subroutine init_data
+
+! the same for an explicit initial condition (ids and parameters of
+! mhd2d_init_data in include/mhd2d.h)
+subroutine init_data_ic(ic, extra, nextra)
   use mhd2d_common
   integer, intent(in) :: ic, nextra
   real(c_double), intent(in) :: extra(*)
   call check(mhd2d_init_data(ctx, int(ic, c_int32_t), extra, int(nextra, c_int32_t)), 'init_data')
-end subroutine init_data
+end subroutine init_data_ic
+
+! calculate_Ez.f90:1-7: the corner electric field Ez = u2*B1 - u1*B2, the
+! same function the vertex kernels evaluate on the device
+subroutine calculate_Ez(Ez, u1, u2, B1, B2)
+  use mhd2d_common
+  real(c_double), intent(out) :: Ez
+  real(c_double), intent(in) :: u1, u2, B1, B2
+  real(c_double) :: a(1), b(1), c(1), d(1), e(1)
+  a(1) = u1
+  b(1) = u2
+  c(1) = B1
+  d(1) = B2
+  call check(mhd2d_calculate_ez(1_c_int64_t, a, b, c, d, e), 'calculate_Ez')
+  Ez = e(1)
+end subroutine calculate_Ez
 
 ! calculate_Az.f90:1-26 (magnetic loop: mode 1 of Bx, By from Az)
 subroutine calculate_Az()
@@ -377,17 +270,20 @@ subroutine load_checkpoint(path)
   call sync_time()
 end subroutine load_checkpoint
 
-! RK3.f90:5-174 without frame output.  t, dt and count stay on the device
-! inside the loop (the step clips dt to tend, and a step that starts at
-! t >= tend changes nothing, exactly as the reference's loop test stops
-! it).  The steps run as HIP graph launches, 16 per read of t; with verbose
-! one per read, and the per-step line of RK3.f90:150 is printed (it costs
-! one extra reduction per step).
+! RK3.f90:5-174.  t, dt and count stay on the device inside the loop (the
+! step clips dt to tend, and a step that starts at t >= tend changes
+! nothing, exactly as the reference's loop test stops it).  The steps run
+! as HIP graph launches, 16 per read of t; with verbose or frame files one
+! per read, and with verbose the per-step line of RK3.f90:151 is printed (it
+! costs one extra reduction per step).  Frames (flash, see mhd2d_common) are
+! the only downloads of the loop.
 subroutine RK3()
   use mhd2d_common
   integer(c_int64_t) :: chunk
+  real(c_double) :: tt
+  integer :: countf
   chunk = 16
-  if (verbose) chunk = 1
+  if (verbose .or. flash == 1) chunk = 1
   call check(mhd2d_rk3_prologue(ctx), 'RK3 prologue')
   call calculate_umax()
   call calculate_totaldiv()
@@ -395,13 +291,24 @@ subroutine RK3()
   t = 0
   count = 0
   if (verbose) print *, t, "  ", umax, "  ", totaldiv
+  if (flash >= 0) then
+    call frame_open()
+    call frame_write()
+  end if
+  tt = tend / frameMAX
+  countf = 1
   do while (t < tend)
     call check(mhd2d_step_graph(ctx, tend, chunk, t), 'RK3 step')
     if (verbose) then
       call check(mhd2d_diagnostics(ctx, umax, totaldiv, pmin), 'RK3 diagnostics')
       print *, t, "  ", umax, "  ", totaldiv
     end if
+    if ((t > tt * countf .or. t == tend) .and. flash == 1) then   ! RK3.f90:154
+      call frame_write()
+      countf = countf + 1
+    end if
   end do
+  if (flash >= 0) call frame_close()
   call sync_time()
 end subroutine RK3
 
@@ -410,7 +317,8 @@ end subroutine RK3
 subroutine RK3_subroutines(max_steps)
   use mhd2d_common
   integer, intent(in) :: max_steps
-  integer :: n
+  integer :: n, countf
+  real(c_double) :: tt
   t = 0
   count = 0
   call check(mhd2d_set_time(ctx, 0d0, 0_c_int64_t), 'set_time')
@@ -421,6 +329,12 @@ subroutine RK3_subroutines(max_steps)
   call calculate_totaldiv()
   call calculate_pmin()
   if (verbose) print *, t, "  ", umax, "  ", totaldiv
+  if (flash >= 0) then
+    call frame_open()
+    call frame_write()
+  end if
+  tt = tend / frameMAX
+  countf = 1
   n = 0
   do while (t < tend .and. (max_steps <= 0 .or. n < max_steps))
     call calculate_dt()
@@ -460,7 +374,12 @@ subroutine RK3_subroutines(max_steps)
       call calculate_pmin()
       print *, t, "  ", umax, "  ", totaldiv
     end if
+    if ((t > tt * countf .or. t == tend) .and. flash == 1) then   ! RK3.f90:154
+      call frame_write()
+      countf = countf + 1
+    end if
   end do
+  if (flash >= 0) call frame_close()
   call check(mhd2d_set_time(ctx, t, count), 'set_time')
 end subroutine RK3_subroutines
 

@@ -12,13 +12,15 @@
 ! condition's initN.txt (mhd2d_ic_defaults).  restart = <file> continues from
 ! a checkpoint instead of projecting the initial condition (no prologue);
 ! checkpoint = <file> writes one at the end; l2err calls calculate_L2error
-! with t20 = tend if tend <= 5, else 0 (main.f90:13-19).
+! with t20 = tend if tend <= 5, else 0 (main.f90:13-19).  frames sets the
+! host's flash (1: Q1..Q8flash.txt and T.txt every tend/frame_max as
+! RK3.f90:29-53,154-172 write them; 0: the t = 0 frame only; -1: none).
 
 program mhd2d_run
   use mhd2d_common
   implicit none
   integer :: Nx_ = 32, Ny_ = 32, flux_type = 4, ic = 103, bc = 1, pp = 0, indicator = 1, precision = 0
-  integer :: max_steps = 0, nextra = 0, u
+  integer :: max_steps = 0, nextra = 0, u, frames = -1, frame_max = 200
   real(c_double) :: xa = 0, xb = 0, ya = 0, yb = 0, M = 10, CFL = 0.15d0, Ck = 1, mlp_lo = 0.2d0, mlp_hi = 0.8d0
   real(c_double) :: tend_ = 0.1d0, extra(64) = 0
   character(len=16) :: mode = 'fused'
@@ -27,9 +29,10 @@ program mhd2d_run
   type(mhd2d_config) :: c
   type(mhd2d_ic_info) :: info
   namelist /run/ Nx_, Ny_, flux_type, ic, bc, pp, indicator, precision, max_steps, xa, xb, ya, yb, M, CFL, Ck, &
-                 mlp_lo, mlp_hi, tend_, extra, nextra, mode, out, qtxt, quiet, restart, checkpoint, l2err
+                 mlp_lo, mlp_hi, tend_, extra, nextra, mode, out, qtxt, quiet, restart, checkpoint, l2err, &
+                 frames, frame_max
   interface
-    subroutine init_data(ic, extra, nextra)
+    subroutine init_data_ic(ic, extra, nextra)
       import :: c_double
       integer, intent(in) :: ic, nextra
       real(c_double), intent(in) :: extra(*)
@@ -71,6 +74,8 @@ program mhd2d_run
   call mhd2d_open(c)
   tend = tend_
   verbose = .not. quiet
+  flash = frames
+  frameMAX = frame_max
 
   if (len_trim(restart) > 0) then
     call load_checkpoint(restart)
@@ -79,7 +84,7 @@ program mhd2d_run
       call sync_time()
     end do
   else
-    call init_data(ic, extra, nextra)
+    call init_data_ic(ic, extra, nextra)
   end if
   if (len_trim(restart) > 0) then
     continue

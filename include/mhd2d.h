@@ -105,7 +105,7 @@ int mhd2d_download(mhd2d_ctx *ctx, double *uh, double *Bx, double *By);
    extra: IC parameters (perturbed OT: amplitude, seed; rotor: r0, r1, spin,
    p, Bx, rho_in, xc, yc; Riemann4: x0, y0, 4 x (rho,u,v,p,Bx,By)). */
 int mhd2d_init_data(mhd2d_ctx *ctx, int32_t ic, const double *extra, int32_t nextra);
-/* Binary checkpoint of the state the time loop carries (RK3.f90:57-172):
+/* Binary checkpoint of the state the time loop carries (RK3.f90:58-174):
    uh, the face fields Bx/By and (t, count), so that a run restarts from
    it bit for bit (the reference writes only uh and has no restart).
    Little-endian file = 128-byte header, payload, CRC-32 trailer:
@@ -131,7 +131,7 @@ int mhd2d_checkpoint_load(mhd2d_ctx *ctx, const char *path);
 /* defaults of one initial condition (initN.txt); -1 for an unknown ic */
 int mhd2d_ic_defaults(int32_t ic, mhd2d_ic_info *out);
 /* calculate_Az.f90:1-26: mode 1 of Bx and By from the magnetic loop's
-   vector potential (Loop_functions.f90:37-52), modes 2-3 untouched */
+   vector potential (Loop_functions.f90:37-51), modes 2-3 untouched */
 int mhd2d_calculate_az(mhd2d_ctx *ctx);
 /* calculate_L2error.f90:1-37: L2 error of each conserved variable against
    the smooth vortex (init1.txt) translated by (t20, t20) */
@@ -141,7 +141,7 @@ int mhd2d_calculate_l2error(mhd2d_ctx *ctx, double t20, double L2[8]);
 int mhd2d_set_bc(mhd2d_ctx *ctx);                        /* set_bc.f90:5-27          */
 int mhd2d_lh(mhd2d_ctx *ctx);                            /* Lh.f90:1-305             */
 int mhd2d_lhez(mhd2d_ctx *ctx);                          /* LhEz.f90:1-96 (after lh) */
-int mhd2d_rk_stage(mhd2d_ctx *ctx, int32_t stage, double dt); /* RK3.f90:157-212     */
+int mhd2d_rk_stage(mhd2d_ctx *ctx, int32_t stage, double dt); /* RK3.f90:74-139      */
 int mhd2d_tvb_limiter_p2(mhd2d_ctx *ctx);                /* TVB_Limiter_P2.f90:1-191 */
 int mhd2d_pp_limiter(mhd2d_ctx *ctx);                    /* pp_Limiter.f90:1-83      */
 int mhd2d_kxrcf_detector(mhd2d_ctx *ctx, int64_t *nflag);/* KXRCF_Detector.f90:1-108 */
@@ -154,11 +154,17 @@ int mhd2d_diagnostics(mhd2d_ctx *ctx, double *umax, double *totaldiv, double *pm
    the face-jump half of the divergence report; div_free_Balsara.f90:43-58
    makes it zero up to rounding) */
 int mhd2d_face_jump(mhd2d_ctx *ctx, double *jump);
+/* calculate_Ez.f90:1-7: Ez[k] = u2[k]*B1[k] - u1[k]*B2[k] for k < n, on the
+   host (the reference's pointwise helper; the vertex kernels evaluate the
+   same function on the device).  Needs no context; -1 on a null pointer or
+   n < 0. */
+int mhd2d_calculate_ez(int64_t n, const double *u1, const double *u2, const double *B1, const double *B2,
+                       double *Ez);
 int mhd2d_rk3_prologue(mhd2d_ctx *ctx);                  /* RK3.f90:8-18             */
-/* one pass of the RK3.f90:135-228 loop body (dt, clip to tend, 3 stages)
+/* one pass of the RK3.f90:58-174 loop body (dt, clip to tend, 3 stages)
    with t and dt kept on the device; t_out (may be NULL) synchronises.  A
    pass that starts at t >= tend changes nothing (the loop test of
-   RK3.f90:135), so passes may be queued without reading t back. */
+   RK3.f90:58), so passes may be queued without reading t back. */
 int mhd2d_step(mhd2d_ctx *ctx, double tend, double *t_out);
 /* nsteps passes of the same loop body, each replayed as one HIP graph
    launch (captured on first use, on one stream, once per phase of the

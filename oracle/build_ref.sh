@@ -17,7 +17,7 @@
 #                           conserved variables (an older equation set) and do
 #                           not compile with this init_data
 #   P5 TVB_Limiter_P2.f90:123  'call pp_Limiter' enabled (variants *_pp)
-#   P7 HLL_Flux_2D.f90:8    EzRU..EzLD are read uninitialised at :176-177;
+#   P7 HLL_Flux_2D.f90:8    EzRU..EzLD are read uninitialised at :61-62;
 #                           they are set to the corner fields v*B1 - u*B2
 #                           (calculate_Ez, as LF_Flux_2D.f90:18-21 does)
 #   P6 KXRCF_Detector.f90   does not compile as shipped (uG undeclared at
@@ -28,7 +28,16 @@
 #                           code never clears a flag).  In variants *_kx the
 #                           commented KXRCF gating of TVB_Limiter_P2.f90:12,
 #                           22,116 is enabled.
-# main.f90 is replaced by oracle/ref_driver.f90.
+# main.f90 is replaced by oracle/ref_driver.f90 in these variants.
+#
+# The drop-in pair (build_main below) keeps the reference's own main.f90 and
+# RK3.f90 untouched, with frame output on (com.txt:1 flash = 1, frameMAX
+# set), the Orszag-Tang include (init_data.f90:19 init3.txt) and a shorter
+# tend (init3.txt:49): ref_main_<name> links the reference's hot-path
+# sources, dropin_<name> links freljord-cfd-arsenal_amd/fortran/
+# mhd2d_com_shims.f90 in their place, over libmhd2d.so (its mhd2d_capi
+# module comes from the package's own build, make -C
+# freljord-cfd-arsenal_amd fortran).
 set -euo pipefail
 
 REF=${REF:-/root/reference/GDF_Balsara_P2_0316}
@@ -122,3 +131,43 @@ echo "$VARIANTS" | while read -r name nx ny ft pp omp kx; do
   [ -z "$name" ] && continue
   build_one "$name" "$nx" "$ny" "$ft" "$pp" "$omp" "${kx:-0}"
 done
+
+PKG=$(cd "$HERE/../freljord-cfd-arsenal_amd" && pwd)
+build_main() {
+  local name=$1 nx=$2 ny=$3 ft=$4 tend=$5 fmax=$6
+  local w
+  w=$(mktemp -d /tmp/gdf_main_XXXXXX)
+  cp "$REF"/*.f90 "$REF"/*.txt "$w"/
+  chmod -R u+w "$w"
+  sed -i "1s/Nx = [0-9]*, Ny = [0-9]*/Nx = $nx, Ny = $ny/; 1s/flux_type = [0-9]*/flux_type = $ft/; 1s/flash = [0-9]*/flash = 1/; 1s/frameMAX = [0-9]*/frameMAX = $fmax/" "$w/com.txt"
+  sed -i "s/include 'init6.txt'/include 'init3.txt'/" "$w/init_data.f90"
+  sed -i "s/^\(\s*\)tend = 3\s*$/\1tend = $tend/" "$w/init3.txt"
+  grep -q "tend = $tend" "$w/init3.txt"
+  local flags="-O2 -fdefault-real-8 -fdefault-double-8"
+  (
+    cd "$w"
+    local host="main RK3 init_data get_GLP get_basis save_XY save_solution calculate_L2error"
+    local hot="set_bc Lh LhEz eigenvalueMm HLL_Flux LF_Flux HLLC_Flux HLLD_Flux HLL_Flux_2D LF_Flux_2D HLLC_Flux_2D
+      HLLD_Flux_2D HLL1 MHD_flux calculate_Ez TVB_Limiter_P2 minmod minmodB eigenmatrix pp_Limiter pressure
+      div_free_Balsara calculate_dt calculate_umax calculate_pmin calculate_totaldiv KXRCF_Detector"
+    sed -i -e "/^\s*real Iij(NumEq),Lij,uinfty(NumEq),Ck/a\\
+    real, save :: uG(Nx,Ny,NumGLP,NumGLP,NumEq)" KXRCF_Detector.f90
+    for f in $host $hot; do echo "$FC $flags -c $f.f90"; done | xargs -P "$JOBS" -I{} sh -c "{}"
+    objs=""
+    for f in $host $hot; do objs="$objs $f.o"; done
+    $FC $objs -o "$OUT/ref_main_$name"
+    cp "$PKG/fortran/mhd2d_com_shims.f90" .
+    $FC $flags -I "$PKG/_build" -c mhd2d_com_shims.f90
+    objs=""
+    for f in $host; do objs="$objs $f.o"; done
+    $FC $objs mhd2d_com_shims.o "$PKG/_build/mhd2d_capi.o" -L"$PKG" -lmhd2d \
+      -Wl,-rpath,'$ORIGIN/../../freljord-cfd-arsenal_amd' -o "$OUT/dropin_$name"
+  )
+  rm -rf "$w"
+  echo "built $OUT/ref_main_$name $OUT/dropin_$name"
+}
+if [ -f "$PKG/_build/mhd2d_capi.mod" ] && [ -f "$PKG/libmhd2d.so" ]; then
+  build_main r16_f4 16 16 4 0.5 10
+else
+  echo "build_ref.sh: the package's Fortran module is not built; drop-in pair skipped" >&2
+fi

@@ -13,7 +13,7 @@
  * bits in most places.  Each function cites the reference file:line it
  * restates.  Deliberate resolutions of reference defects:
  *   - Lh.f90:268-298 races under OpenMP; this sums each du element serially.
- *   - HLL_Flux_2D.f90:8,176-177 reads EzRU..EzLD uninitialised; here they are
+ *   - HLL_Flux_2D.f90:8,61-62 reads EzRU..EzLD uninitialised; here they are
  *     the corner electric fields v*B1 - u*B2 (calculate_Ez.f90:5), as
  *     LF_Flux_2D.f90:18-21 computes them.
  *   - KXRCF_Detector.f90 does not compile (uG undeclared) and never resets
@@ -149,7 +149,7 @@ static void get_basis(tables_t *T, double hx1, double hy1) {
         T->phiyG[d][i][j] = dy[d];
       }
     }
-    /* face traces, get_basis.f90:51-243 (index i plays j1 or i1) */
+    /* face traces, get_basis.f90:49-160 (index i plays j1 or i1) */
     double li = l[i], P2 = li * li - t3, P3 = li * li * li - 3.0 * li / 5.0;
     double R[NP] = {1, 1, li, t23, li, P2, t25, t23 * (li), P2, P3};
     double Lf[NP] = {1, -1, li, t23, -li, P2, -t25, t23 * (li), -P2, P3};
@@ -885,14 +885,14 @@ static void ic_eval(const icdef_t *c, double x, double y, double U[8]) {
       B1 = 100.0 / pow(4 * pi, 0.5);
       break;
     }
-    case ORC_IC_CLOUD_SHOCK: { /* init6.txt:3-18, Cloud_Shock_functions.f90:1-118 */
+    case ORC_IC_CLOUD_SHOCK: { /* init6.txt:3-18, Cloud_Shock_functions.f90:1-117 */
       double r = pow((x - 1.4) * (x - 1.4) + (y - 0.5) * (y - 0.5), 0.5);
       if (x <= 1.2) { rho = 3.88968; v3 = -0.05234; p = 14.2641; B3 = 3.9353; }
       else { rho = r >= 0.18 ? 1.0 : 5.0; v1 = -3.3156; p = 0.04; B3 = 1.0; }
       B1 = 1.0;
       break;
     }
-    case ORC_IC_CLOUD_SHOCK_2: { /* init6b.txt:3-18, Cloud_Shock_functions2.f90:1-118 */
+    case ORC_IC_CLOUD_SHOCK_2: { /* init6b.txt:3-18, Cloud_Shock_functions2.f90:1-116 */
       double r = pow((x - 0.25) * (x - 0.25) + (y - 0.5) * (y - 0.5), 0.5);
       if (x <= 0.05) { rho = 3.88968; v1 = 11.2536; p = 167.345; B2 = 0.21826182; B3 = -0.21826182; }
       else { rho = r >= 0.15 ? 1.0 : 10.0; p = 1.0; B2 = 0.56418958; B3 = 0.56418958; }
@@ -1039,7 +1039,7 @@ void orc_init_data(orc_state *s, int ic, const double *extra, int nextra) {
     }
 }
 
-/* Loop_functions.f90:37-52 */
+/* Loop_functions.f90:37-51 */
 static double loop_Az(double x, double y) {
   double r = pow(x * x + y * y, 0.5);
   return r < 0.3 ? 1e-3 * (0.3 - r) : 0.0;
@@ -1636,7 +1636,7 @@ static void lin2(double *out, double a, const double *x, double b, const double 
   for (size_t k = 0; k < n; k++) out[k] = a * x[k] + b * y[k];
 }
 
-/* RK3.f90:150-216 */
+/* RK3.f90:74-139 (the three stages) */
 void orc_rk3_stages(orc_state *s, double dt) {
   size_t nu = s->nuh, nx = s->nbx, ny = s->nby;
   /* stage 1 */
@@ -1683,7 +1683,7 @@ void orc_rk3_stages(orc_state *s, double dt) {
   orc_div_free_Balsara(s);
 }
 
-/* RK3.f90:135-228 (one pass of the time loop) */
+/* RK3.f90:58-174 (one pass of the time loop, without frame output) */
 int orc_rk3_step(orc_state *s, double tend) {
   if (!(s->t < tend)) return 0;
   orc_calculate_dt(s);

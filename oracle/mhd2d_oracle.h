@@ -81,7 +81,7 @@ void orc_calculate_totaldiv(orc_state *s);
 
 /* RK3.f90:8-18 (limit + diagnostics before the loop) */
 void orc_rk3_prologue(orc_state *s);
-/* one pass of the RK3.f90:58-150 loop body: dt, clip to tend, 3 stages,
+/* one pass of the RK3.f90:58-174 loop body: dt, clip to tend, 3 stages,
    diagnostics.  Returns 1 if a step was taken, 0 if t >= tend. */
 int orc_rk3_step(orc_state *s, double tend);
 /* the three stages with a given dt (no dt computation, no clip) */

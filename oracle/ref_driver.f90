@@ -84,7 +84,7 @@ program ref_driver
         call RK3
     case ('bench')
         ! the RK3.f90 prologue and nsteps passes of its loop body, with the
-        ! array combinations of RK3.f90:157-212; no frame output
+        ! array combinations of RK3.f90:80-135; no frame output
         t = 0
         count = 0
         call set_bc

@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from conftest import PKG_DIR, ROOT
@@ -68,3 +69,26 @@ def test_ic_defaults_match_reference(pkg, tag, ic):
         assert d["bc"] == (int(g["bc"]),) * 4
     with pytest.raises(pkg.MHD2DError):
         pkg.ic_defaults(12345)
+
+
+def test_fortran_binding_covers_every_export():
+    """fortran/mhd2d_capi.f90 has one bind(C) interface per function the header
+    declares, under the same C name"""
+    f = open(os.path.join(PKG_DIR, "fortran", "mhd2d_capi.f90")).read()
+    bound = sorted(set(re.findall(r"name='(mhd2d_\w+)'", f)))
+    assert bound == declared_functions()
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libmhd2d.so not built")
+def test_calculate_ez_closed_form(pkg):
+    """calculate_Ez.f90:5, Ez = u2*B1 - u1*B2: zero for u parallel to B, odd
+    in the velocity, and the reference's product order (host entry point, no
+    device needed)"""
+    rng = np.random.default_rng(5)
+    u1, u2, B1, B2 = rng.standard_normal((4, 257))
+    ez = pkg.calculate_Ez(u1, u2, B1, B2)
+    assert np.array_equal(ez, u2 * B1 - u1 * B2)
+    assert np.array_equal(pkg.calculate_Ez(-u1, -u2, B1, B2), -ez)
+    assert np.all(pkg.calculate_Ez(B1, B2, B1, B2) == 0.0)
+    lib = ctypes.CDLL(LIB)
+    assert lib.mhd2d_calculate_ez(ctypes.c_int64(-1), None, None, None, None, None) == -1

@@ -117,3 +117,33 @@ def test_host_l2error_vortex(tmp_path, pkg):
     assert L2_host.shape == (8,)
     assert np.allclose(L2_host, L2, rtol=1e-12, atol=1e-300)
     assert L2[0] > 0 and L2[3] == 0.0 and L2[7] == 0.0
+
+
+def test_host_frames_vs_reference(tmp_path):
+    """the host's RK3 with frames on (frames=1, frame_max=10) writes the frame
+    files of RK3.f90:29-53,154-172: Q1..Q8flash.txt, one value per line in
+    d -> j -> i order, and T.txt.  Against the reference's own main program
+    (oracle/_ref/ref_main_r16_f4: Orszag-Tang 16 x 16, HLLD, M = 10, TVB on
+    every cell, CFL 0.15, tend = 0.5, frameMAX = 10) every frame agrees to
+    1e-12 and the frame times to 1e-14"""
+    from test_dropin_reference import frames, numbers, run_program
+    ref = tmp_path / "ref"
+    ref.mkdir()
+    run_program("ref_main_r16_f4", ref)
+    host = tmp_path / "host"
+    host.mkdir()
+    nml = host / "frames.nml"
+    nml.write_text("&run Nx_=16, Ny_=16, flux_type=4, ic=3, pp=0, indicator=1, CFL=0.15d0, tend_=0.5d0, "
+                   "frames=1, frame_max=10, qtxt=.true., quiet=.true. /\n")
+    print("[fortran host] frames vs the reference main program", flush=True)
+    r = subprocess.run([EXE, str(nml)], cwd=host, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ta, tb = numbers(ref / "T.txt"), numbers(host / "T.txt")
+    assert ta.size == tb.size >= 5 and np.allclose(tb, ta, rtol=1e-14, atol=0)
+    for n in range(1, 9):
+        fa, fb = frames(str(ref), n), frames(str(host), n)
+        assert fa.shape == fb.shape
+        for k in range(fa.shape[0]):
+            assert rel_l2(fb[k], fa[k]) <= 1e-12, (n, k)
+        # the last frame (t = tend) is the final state save_solution writes
+        assert np.array_equal(fb[-1].ravel(), numbers(host / f"Q{n}.txt"))

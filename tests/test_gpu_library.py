@@ -127,6 +127,38 @@ def test_checkpoint_format(pkg, tmp_path):
     other = pkg.Solver(16, 16)
     with pytest.raises(pkg.MHD2DError, match="grid"):
         other.load_checkpoint(str(path))
+    # a context with another domain, boundaries or Riemann solver rejects it
+    dom = (0.0, 2 * np.pi, 0.0, 2 * np.pi)
+    for kw, why in (({"domain": (0.0, 1.0, 0.0, 1.0)}, "domain"),
+                    ({"domain": dom, "bc": (2, 2, 2, 2)}, "boundary"),
+                    ({"domain": dom, "flux_type": pkg.HLL}, "flux_type")):
+        args = dict(flux_type=pkg.HLLD, M=50.0, CFL=0.2, pp=1, indicator=pkg.IND_KXRCF, Ck=1.0)
+        args.update(kw)
+        o = pkg.Solver(N, N, **args)
+        with pytest.raises(pkg.MHD2DError, match=why):
+            o.load_checkpoint(str(path))
+
+
+def test_checkpoint_of_a_flagged_run(pkg, tmp_path):
+    """a numerical flag (a positive status of the time read) does not stop
+    the save: the file is written and restarts bit for bit, flag included
+    in nothing but the status"""
+    s = _run(pkg, pkg.FP64, pkg.IND_KXRCF, 2)
+    uh, Bx, By = s.download()
+    neg = uh.copy()
+    neg[7, 7, 0, 4] = 0.1 * neg[7, 7, 0, 4] - 5.0   # negative pressure at this cell
+    s.upload(neg, Bx, By)
+    assert s.diagnostics()["flags"] & pkg.FLAG_NEG_PRESSURE
+    path = tmp_path / "flagged.bin"
+    s.save_checkpoint(str(path))
+    assert path.exists() and path.stat().st_size > 128
+    r = pkg.Solver(32, 32, flux_type=pkg.HLLD, domain=(0.0, 2 * np.pi, 0.0, 2 * np.pi), M=50.0, CFL=0.2, pp=1,
+                   indicator=pkg.IND_KXRCF, Ck=1.0)
+    r.load_checkpoint(str(path))
+    ur, xr, yr = r.download()
+    us, xs, ys = s.download()
+    assert np.array_equal(ur, us) and np.array_equal(xr, xs) and np.array_equal(yr, ys)
+    assert r.get_time()[2] == s.get_time()[2] == 2
 
 
 def test_checkpoint_version2_mixed(pkg, tmp_path):

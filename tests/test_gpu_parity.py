@@ -245,7 +245,7 @@ def test_rk3_per_subroutine(pkg, oracle_mod, name, ft, pp):
                 s.set_bc(); o.set_bc()
             s.Lh(); s.LhEz(); s.rk_stage(stage, dt)
             o.Lh(); o.LhEz()
-            # the oracle's combination for this stage (RK3.f90:157-212)
+            # the oracle's combination for this stage (RK3.f90:80-135)
             _oracle_stage(o, stage, dt)
             check(f"step {step} stage {stage} combine")
             s.TVB_Limiter_P2(); s.div_free_Balsara()
