@@ -30,12 +30,28 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 DOF_PER_CELL = 48
 
-# algorithmic bytes per interior cell for one launch of each kernel class.
-# State: 52 FP64 planes (416 B/cell) in the plain form, 296 B/cell in the
-# mixed storage; face B: 6 FP64 planes (48 B/cell).  Face fluxes (40 values per face), face EMFs (5 per face)
-# are FP64 in the plain form and FP32 in the mixed form (bytes B below).
-# One x face and one y face belong to each cell.
-def alg_bytes(form):
+# Algorithmic bytes of the step (the roofline's numerator).  The independent
+# state is 42 values per cell: 36 cell modes (6 variables x 6 modes; the
+# in-cell Bx, By modes are derived from the faces by div_free_Balsara) and
+# 6 face-B modes.  SSP-RK3 keeping U^n moves 8 state-sized transfers per
+# step (stage 1: read U^n, write U1; stage 2: read U1 and U^n, write U2;
+# stage 3: read U2 and U^n, write U^n+1).  Plain FP64: 42 x 8 B x 8 =
+# 2688 B per cell per step.  The mixed storage holds the 6 cell averages
+# and the 6 face modes in FP64 and the 30 higher modes in FP32: 216 B per
+# cell, 1728 B per cell per step.  A stage is a third of a step.
+STEP_ALG_BYTES = {"fp64": 42 * 8 * 8, "mixed": (12 * 8 + 30 * 4) * 8}
+STAGE_ALG_BYTES = {k: v / 3.0 for k, v in STEP_ALG_BYTES.items()}
+
+
+# Bytes each kernel class reads and writes per interior cell and launch as
+# this implementation lays them out (not the roofline's numerator: the face
+# moments' round trip, halo re-reads and the separate passes are
+# implementation traffic).  State: 52 FP64 planes (416 B/cell) in the plain
+# form, 296 B/cell in the mixed storage; face B: 6 FP64 planes (48 B/cell).
+# Face fluxes (40 values per face), face EMFs (5 per face) are FP64 in the
+# plain form and FP32 in the mixed form (bytes B below).  One x face and one
+# y face belong to each cell.
+def io_bytes(form):
     B = 8 if form == "fp64" else 4
     # one state: 52 FP64 planes; mixed storage: 22 FP64 planes (cell
     # averages, magnetic modes) + 30 FP32 planes (higher modes of the rest)
@@ -67,7 +83,6 @@ def alg_bytes(form):
     }
 
 
-STEP_ALG_BYTES_PER_CELL = 2688     # 42 independent FP64 values x 8 state transfers per SSP-RK3 step
 # kernel of each class in the traffic summary (the limiter class also runs
 # the detector and the sparse TVB pass, whose traffic is listed on its own)
 KERNEL_SYMBOL = {"xface": "k_xface", "yface": "k_yface", "vertex": "k_vertex", "cell": "k_cell", "ct": "k_ct",
@@ -106,14 +121,26 @@ def import_pkg():
     return mod
 
 
-def cpu_baseline(N=256, steps=2):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(N=512, steps=1):
     """the reference Fortran (OpenMP build in oracle/_ref) on a bounded
-    sample of the same workload: N x N cells, `steps` RK3 steps"""
+    sample of the same workload and method (TVB gated by KXRCF, positivity,
+    HLLD): N x N cells, `steps` RK3 steps"""
     from oracle import oracle as O
-    variant = f"r{N}_f4_pp_omp"
+    variant = f"r{N}_f4_pp_kx_omp"
     if not O.ref_available(variant):
         return None
-    threads = 16    # Lh.f90:9 requests num_threads(16); the rest of the step is serial
+    # the box's CPU share for one GPU is 16 threads; Lh.f90:9 asks for 16
+    threads = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
     o = O.Oracle(N, N, flux_type=O.HLLD, domain=(0.0, 2 * np.pi, 0.0, 2 * np.pi), M=50.0, CFL=0.2, pp=1,
                  nthreads=1)
     o.init_data(O.IC_OT_PERTURBED, [1e-2, 20261203])
@@ -126,9 +153,12 @@ def cpu_baseline(N=256, steps=2):
         os.environ.update(env_backup)
     secs = float(r["count"])      # the bench mode returns the timed seconds in this slot
     return {"value": DOF_PER_CELL * N * N * steps / secs, "unit": "DOF-updates/s", "cores": threads,
-            "kind": "reference",
-            "sample": f"reference Fortran (flang -O3 -fopenmp, OpenMP only in Lh) on the same workload at "
-                      f"{N}x{N}, {steps} RK3 steps (CFL 0.15 as shipped), {secs:.1f} s"}
+            "kind": "reference", "host_logical_cpus": os.cpu_count(), "cpu_model": cpu_model(),
+            "sample": f"reference Fortran (flang -O3 -fopenmp) on the same workload and method (perturbed "
+                      f"Orszag-Tang, HLLD, TVB M=50 gated by KXRCF Ck=1, positivity) at {N}x{N}, {steps} RK3 "
+                      f"steps, {secs:.1f} s, OMP_NUM_THREADS={threads}; only Lh is OpenMP-parallel "
+                      f"(Lh.f90:9-300), LhEz, the limiters, the RK statements and the reductions run on one "
+                      f"thread; per-step cost does not depend on the CFL (0.15 as shipped)"}
 
 
 def counters_run(pkg, N, dom):
@@ -186,36 +216,50 @@ def rel_l2_fields(a, b):
     return out
 
 
-def copy_ceiling_gbs(nbytes=4 << 30, reps=10):
-    """HBM bandwidth of a plain device-to-device hipMemcpy on this GPU (read +
-    write bytes over the mean copy time): the measured ceiling beside the
-    8 TB/s specification"""
-    import ctypes as C
-    hip = C.CDLL("libamdhip64.so")
-    a, b, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
-    n = C.c_size_t(nbytes)
-    try:
-        if hip.hipMalloc(C.byref(a), n) or hip.hipMalloc(C.byref(b), n):
-            return None
-        hip.hipEventCreate(C.byref(e0))
-        hip.hipEventCreate(C.byref(e1))
-        hip.hipMemcpyAsync(b, a, n, 3, None)          # 3 = hipMemcpyDeviceToDevice
-        hip.hipDeviceSynchronize()
-        hip.hipEventRecord(e0, None)
-        for _ in range(reps):
-            hip.hipMemcpyAsync(b, a, n, 3, None)
-        hip.hipEventRecord(e1, None)
-        hip.hipEventSynchronize(e1)
-        ms = C.c_float()
-        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
-        return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9 if ms.value > 0 else None
-    finally:
-        for p in (a, b):
-            if p.value:
-                hip.hipFree(p)
-        for e in (e0, e1):
-            if e.value:
-                hip.hipEventDestroy(e)
+def triad_ceiling(pkg, n=1 << 28, reps=10):
+    """the measured HBM ceiling on this GPU: the in-tree triad kernel
+    (mhd2d_stream_triad) over three arrays of n doubles (6 GiB per launch),
+    best over its unroll variants"""
+    best = None
+    for u in (1, 2, 4, 8):
+        gbs, ms = pkg.stream_triad(n, u, reps)
+        if best is None or gbs > best["gbs"]:
+            best = {"gbs": gbs, "unroll": u, "ms_per_launch": ms, "bytes_per_launch": 24 * n}
+    return best
+
+
+def error_run(pkg, N, tend, dom, nn):
+    """the error statement past the onset of limiting: the plain FP64 method
+    and the headline form advanced from the same input to t = tend at N^2
+    (queued steps until t reaches tend), per-field relative L2 of the mixed
+    state against the FP64 one, both forms' div B, and the indicator
+    statistics of the FP64 run's last limiter input"""
+    out = {"N": N, "t_end": tend}
+    states = {}
+    for name, prec, ind in (("fp64", pkg.FP64, pkg.IND_KXRCF), ("mixed_nn", pkg.MIXED, pkg.IND_MLP)):
+        if name == "mixed_nn" and nn is None:
+            continue
+        band = (nn[1], nn[2]) if nn else (0.2, 0.8)
+        s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=ind,
+                       precision=prec, Ck=1.0, mlp_band=band)
+        if nn:
+            s.mlp_set_weights(nn[0])
+        s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
+        t0 = time.perf_counter()
+        t, steps = s.RK3(tend)
+        secs = time.perf_counter() - t0
+        d = s.diagnostics()
+        states[name] = s.download()[0]
+        out[name] = {"steps": steps, "t": t, "seconds": secs, "totaldiv": d["totaldiv"], "pmin": d["pmin"],
+                     "face_jump": s.face_jump()}
+        if name == "mixed_nn":
+            out[name]["indicator_stats"] = indicator_mod().limiter_input_stats(s)
+        s.close()
+        print(f"[bench] error run {name}: {steps} steps to t={t:.4f} at {N}^2 in {secs:.1f} s",
+              file=sys.stderr, flush=True)
+    if "mixed_nn" in states:
+        out["rel_l2_vs_fp64"] = rel_l2_fields(states["mixed_nn"], states["fp64"])
+    return out
 
 
 def main():
@@ -231,6 +275,9 @@ def main():
                     help="also time the plain form with TVB on every cell (the reference as shipped)")
     ap.add_argument("--error", type=int, default=1, help="report mixed-vs-FP64 L2 errors (downloads both states)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--error-n", type=int, default=1024,
+                    help="grid of the longer error run (both forms to --error-t, through the shocks); 0 = off")
+    ap.add_argument("--error-t", type=float, default=0.5)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--graph", action="store_true",
                     help="time HIP-graph replays of the step (mhd2d_step_graph, one stream) instead of queued "
@@ -326,14 +373,21 @@ def main():
     if args.form == "mixed_nn" and args.error and "mixed" in results:
         # what the learned screen alone changes: same mixed arithmetic, KXRCF on every cell
         err_screen = rel_l2_fields(head["state"][0], results["mixed"]["state"][0])
+    storage = "fp64" if args.form == "fp64" else "mixed"
     dom_k = max(prof, key=lambda k: prof[k][0])
     ms_tot, launches = prof[dom_k]
     avg_ms = ms_tot / max(launches, 1)
-    bytes_launch = alg_bytes("fp64" if args.form == "fp64" else "mixed")[dom_k] * N * N
+    # roofline numerator: the algorithmic bytes of one RK stage (the dominant
+    # kernel runs once per stage) over its mean launch time
+    bytes_launch = STAGE_ALG_BYTES[storage] * N * N
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    step_gbs = STEP_ALG_BYTES_PER_CELL * N * N / (ms_per_step * 1e-3) / 1e9
-    copy_gbs = copy_ceiling_gbs()
+    io_launch = io_bytes(storage)[dom_k] * N * N
+    step_gbs = STEP_ALG_BYTES[storage] * N * N / (ms_per_step * 1e-3) / 1e9
+    fp64_step_gbs = STEP_ALG_BYTES["fp64"] * N * N / (results["fp64"]["ms"] * 1e-3) / 1e9
+    traffic = measured_traffic(dom_k, args.form, N)
+    ceiling = triad_ceiling(pkg)
     cpu = None if args.no_cpu else cpu_baseline()
+    err_run = error_run(pkg, args.error_n, args.error_t, dom, nn) if (args.error_n and args.form != "fp64") else None
     per_step = lambda r: {k: v[0] / args.profile_steps for k, v in r["prof"].items() if v[1]}
     out = {
         "metric": "P2 MHD DOF-updates/s @2048^2: mixed+NN vs plain FP64, same GPU, at stated L2 err",
@@ -364,11 +418,20 @@ def main():
                    "indicator_stats": head.get("istats"),
                    "mlp_band": [nn[1], nn[2]] if nn else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_k, args.form, N),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": dom_k, "alg_bytes_per_launch": bytes_launch,
+                     "alg_bytes_per_cell_stage": STAGE_ALG_BYTES[storage],
+                     "traffic_over_alg": traffic / bytes_launch if traffic else None,
+                     "kernel_io_bytes_per_launch": io_launch,
                      "kernel_avg_ms": avg_ms, "step_achieved": step_gbs,
-                     "step_frac": step_gbs / HBM_PEAK_GBS, "copy_ceiling": copy_gbs,
-                     "frac_of_copy": achieved / copy_gbs if copy_gbs else None},
+                     "step_alg_bytes_per_cell": STEP_ALG_BYTES[storage],
+                     "step_frac": step_gbs / HBM_PEAK_GBS,
+                     "fp64_step_achieved": fp64_step_gbs, "fp64_step_frac": fp64_step_gbs / HBM_PEAK_GBS,
+                     "ceiling": ceiling["gbs"], "ceiling_kernel": f"mhd2d_stream_triad (unroll "
+                                                                  f"{ceiling['unroll']}, 6 GiB per launch)",
+                     "frac_of_ceiling": achieved / ceiling["gbs"],
+                     "step_frac_of_ceiling": step_gbs / ceiling["gbs"]},
+        "error_run": err_run,
         "cpu_baseline": cpu,
         "kernels_ms_per_step": per_step(head),
         "kernels_ms_per_step_by_run": {k: per_step(r) for k, r in results.items()},

@@ -42,7 +42,7 @@ EXPORTS = (
     "mhd2d_trainer_size", "mhd2d_trainer_clear", "mhd2d_trainer_get_data", "mhd2d_trainer_set_data",
     "mhd2d_trainer_grad", "mhd2d_trainer_fit",
     "mhd2d_checkpoint_save", "mhd2d_checkpoint_load", "mhd2d_ic_defaults", "mhd2d_calculate_az",
-    "mhd2d_calculate_l2error", "mhd2d_calculate_ez",
+    "mhd2d_calculate_l2error", "mhd2d_calculate_ez", "mhd2d_stream_triad",
 )
 MLP_NPARAM = 8129        # W1[60][64], b1[64], W2[64][64], b2[64], w3[64], b3
 MLP_NIN = 60
@@ -136,6 +136,7 @@ def load_library(path: str = LIB_PATH):
         "mhd2d_calculate_az": ([P], C.c_int),
         "mhd2d_calculate_l2error": ([P, C.c_double, dp], C.c_int),
         "mhd2d_calculate_ez": ([C.c_int64, dp, dp, dp, dp, dp], C.c_int),
+        "mhd2d_stream_triad": ([C.c_int64, i32, i32, dp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -187,6 +188,17 @@ def calculate_Ez(u1, u2, B1, B2):
     if rc != 0:
         raise MHD2DError(f"mhd2d_calculate_ez failed with status {rc}")
     return out
+
+
+def stream_triad(n=1 << 28, unroll=4, reps=10):
+    """measured HBM ceiling: (GB/s, ms per launch) of a triad over three
+    arrays of n doubles (mhd2d_stream_triad)"""
+    L = load_library()
+    g, ms = C.c_double(), C.c_double()
+    rc = L.mhd2d_stream_triad(int(n), int(unroll), int(reps), C.byref(g), C.byref(ms))
+    if rc != 0:
+        raise MHD2DError(f"mhd2d_stream_triad failed with status {rc}")
+    return g.value, ms.value
 
 
 class Solver:

@@ -196,6 +196,13 @@ module mhd2d_capi
       integer(c_int64_t), intent(out) :: launches(*)
       integer(c_int32_t), value :: nclass
     end function
+    integer(c_int) function mhd2d_stream_triad(n, unroll, reps, gbs, ms_per_launch) &
+        bind(C, name='mhd2d_stream_triad')
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int64_t), value :: n
+      integer(c_int32_t), value :: unroll, reps
+      real(c_double), intent(out) :: gbs, ms_per_launch
+    end function
     ! learned troubled-cell indicator
     integer(c_int) function mhd2d_mlp_set_weights(ctx, params, n) bind(C, name='mhd2d_mlp_set_weights')
       import :: c_int, c_ptr, c_int32_t, c_float

@@ -188,6 +188,13 @@ int mhd2d_debug_get(mhd2d_ctx *ctx, int32_t which, double *host);
    5 limiter, 6 set_bc, 7 dt, 8 KXRCF, 9 diagnostics. */
 int mhd2d_profile_enable(mhd2d_ctx *ctx, int32_t on);
 int mhd2d_profile_get(mhd2d_ctx *ctx, double *ms, int64_t *launches, int32_t nclass);
+/* The measured HBM ceiling (no counterpart in the reference; for
+   benchmarks): a STREAM triad a = b + 1.5 c over three arrays of n doubles
+   (n even; the library allocates and frees them), `unroll` (1, 2, 4 or 8)
+   16-byte elements per lane, timed with HIP events over `reps` launches
+   after one warm-up.  gbs = 24 n bytes / mean launch time; ms_per_launch
+   may be NULL.  Needs no context. */
+int mhd2d_stream_triad(int64_t n, int32_t unroll, int32_t reps, double *gbs, double *ms_per_launch);
 
 /* ---- learned troubled-cell indicator (no counterpart in the reference) ----
    A 60-64-64-1 MLP (GELU hidden layers, sigmoid output) over a cell's and its

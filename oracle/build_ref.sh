@@ -64,7 +64,7 @@ r32_f4_pp 32 32 4 1 0 0
 r32_f2 32 32 2 0 0 0
 r32_f4_pp_kx 32 32 4 1 0 1
 r32_f2_kx 32 32 2 0 0 1
-r256_f4_pp_omp 256 256 4 1 1 0
+r512_f4_pp_kx_omp 512 512 4 1 1 1
 "}
 CK=${CK:-1}
 
