@@ -220,7 +220,7 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
     const int p = P;
     const bool avg = p < 48 && (p % 6) == 0;
     const int b = p < 48 ? p % 6 : (p == 48 ? 6 : p == 49 ? 8 : p == 50 ? 7 : 9);
-    const double du = avg ? acc0[p / 6] : (double)acc[p] * (1.0 / mmass(b));
+    const double du = avg ? acc0[avg ? p / 6 : 0] : (double)acc[p] * (1.0 / mmass(b));
     double r;
     if (MODE == 0) r = du;
     // the LDS copy is exact for the planes stored in the quadrature type

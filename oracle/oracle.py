@@ -19,6 +19,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libmhd2d_oracle.so")
+# the FMA-contracted build of the same source (see Makefile)
+FMA_LIB_PATH = os.path.join(HERE, "_build", "libmhd2d_oracle_fma.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
 LF, HLL, HLLC, HLLD = 1, 2, 3, 4
@@ -39,6 +41,7 @@ class OrcParams(C.Structure):
 
 
 _lib = None
+_libs = {}
 
 
 def build():
@@ -46,12 +49,25 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib():
+def lib(variant=""):
+    """the restatement's library; variant "fma" is the FMA-contracted build"""
     global _lib
+    if variant:
+        if variant not in _libs:
+            path = {"fma": FMA_LIB_PATH}[variant]
+            if not os.path.exists(path):
+                build()
+            _libs[variant] = _declare(C.CDLL(path))
+        return _libs[variant]
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
+        _lib = _declare(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def _declare(L):
+    if True:
         P = C.c_void_p
         dp = C.POINTER(C.c_double)
         L.orc_create.restype = P
@@ -82,8 +98,7 @@ def lib():
         L.orc_vertex_emf.restype = C.c_double
         L.orc_eigenmatrix.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double]
         L.orc_mhd_flux.argtypes = [dp, dp, dp]
-        _lib = L
-    return _lib
+    return L
 
 
 def _dp(a):
@@ -94,13 +109,13 @@ class Oracle:
     """One restatement state (the reference's COMMON blocks)."""
 
     def __init__(self, Nx, Ny=None, flux_type=HLLD, bc=1, domain=(0.0, 2 * np.pi, 0.0, 2 * np.pi),
-                 M=10.0, CFL=0.15, pp=0, indicator=IND_ALL, Ck=1.0, nthreads=0):
+                 M=10.0, CFL=0.15, pp=0, indicator=IND_ALL, Ck=1.0, nthreads=0, variant=""):
         Ny = Nx if Ny is None else Ny
         self.Nx, self.Ny = Nx, Ny
         bcs = (bc, bc, bc, bc) if np.isscalar(bc) else tuple(bc)
         self.params = OrcParams(Nx, Ny, flux_type, *bcs, pp, indicator, nthreads,
                                 *domain, M, CFL, Ck)
-        self.L = lib()
+        self.L = lib(variant)
         self.h = self.L.orc_create(C.byref(self.params))
 
     def __del__(self):

@@ -94,3 +94,77 @@ def test_vortex_convergence(pkg):
     assert abs(emx[0] - e64[0]) < 1e-2 * e64[0]
     assert max(dmx) < 1e-5
     assert all(r["divB_fp64"] < 1e-12 and r["divB_mixed"] < 1e-12 for r in rows)
+
+
+def test_plain_1024_vs_oracle(pkg, oracle_mod):
+    """the plain method's own configuration at its size: perturbed
+    Orszag-Tang 1024^2, HLLD, TVB (M = 50) gated by KXRCF, positivity, CFL
+    0.2, three fused GPU steps against three oracle steps from the same
+    projected input (the oracle is bit-identical to the reference on every
+    fixture): 1e-12 relative L2 in the state and the faces, equal t"""
+    O = oracle_mod
+    N, dom, steps = 1024, (0.0, TWO_PI, 0.0, TWO_PI), 3
+    o = O.Oracle(N, N, flux_type=O.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=O.IND_KXRCF, Ck=1.0,
+                 nthreads=16)
+    o.init_data(O.IC_OT_PERTURBED, [1e-2, 20261203])
+    s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=pkg.IND_KXRCF, Ck=1.0)
+    s.upload(o.uh, o.Bx, o.By)
+    o.rk3_prologue()
+    s.rk3_prologue()
+    for k in range(steps):
+        print(f"[configs] 1024^2 oracle step {k + 1} of {steps}", flush=True)
+        o.rk3_step(1e9)
+        s.step()
+    uh, Bx, By = s.download()
+    err = rel_l2(interior(uh), interior(o.uh))
+    print(f"[configs] perturbed OT 1024^2 HLLD+pp+KXRCF, {steps} steps: rel L2 vs oracle {err:.2e}")
+    assert s.get_time()[0] == o.scalars()["t"]
+    assert err < 1e-12 and rel_l2(Bx, o.Bx) < 1e-12 and rel_l2(By, o.By) < 1e-12
+    assert np.array_equal(s.debug_get(pkg.TROUBLE) != 0, o.trouble != 0)
+
+
+def test_combined_4096(pkg):
+    """the combined workload: seeded perturbed Orszag-Tang 4096^2 (805 M DOF),
+    HLLD, mixed precision with the learned screen (KXRCF on its candidates),
+    200 graph-replayed RK3 steps, against the plain FP64 method over the same
+    200 steps: relative L2 <= 1e-5 in every field, div B <= 1e-12 in the
+    mixed form, mass conserved, finite state, and the screen's decisions a
+    subset of the plain method's"""
+    N, dom, steps = 4096, (0.0, TWO_PI, 0.0, TWO_PI), 200
+    z = np.load(pkg.WEIGHTS_PATH, allow_pickle=False)
+    band = (float(z["mlp_lo"]), float(z["mlp_hi"]))
+    res = {}
+    for name, prec, ind in (("fp64", pkg.FP64, pkg.IND_KXRCF), ("mixed_nn", pkg.MIXED, pkg.IND_MLP)):
+        s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, indicator=ind, precision=prec,
+                       Ck=1.0, mlp_band=band)
+        s.mlp_set_weights(pkg.load_indicator_weights())
+        s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
+        s.rk3_prologue()
+        m0 = float(interior(s.download()[0])[:, :, 0, 0].sum())
+        s.step_graph(0)                      # capture
+        for k in range(0, steps, 50):
+            s.step_graph(50)
+            print(f"[configs] 4096^2 {name}: {k + 50} of {steps} steps", flush=True)
+        uh = interior(s.download()[0])[:, :, :6, :].copy()
+        d = s.diagnostics()
+        res[name] = dict(uh=uh, d=d, t=s.get_time(), m0=m0, jump=s.face_jump(),
+                         stats=s.indicator_stats() if name == "mixed_nn" else None)
+        s.close()
+    a, b = res["mixed_nn"], res["fp64"]
+    assert a["t"][2] == b["t"][2] == steps and abs(a["t"][0] - b["t"][0]) < 1e-6 * b["t"][0]
+    w = np.sqrt(np.array([1.0, 1 / 3, 1 / 3, 4 / 45, 1 / 9, 4 / 45]))[None, None, :]
+    errs = {}
+    for n, nm in ((0, "rho"), (1, "rhou"), (2, "rhov"), (4, "E"), (5, "Bx"), (6, "By")):
+        ref = b["uh"][:, :, :, n] * w
+        errs[nm] = float(np.linalg.norm(a["uh"][:, :, :, n] * w - ref) / np.linalg.norm(ref))
+    st = a["stats"]
+    print(f"[configs] combined 4096^2, {steps} steps: rel L2 vs FP64 {errs}, div B {a['d']['totaldiv']:.2e} "
+          f"(FP64 {b['d']['totaldiv']:.2e}), face jump {a['jump']:.1e}, screen {st}")
+    assert max(errs.values()) <= 1e-5
+    assert a["d"]["totaldiv"] <= 1e-12 and a["jump"] <= 1e-13
+    for r in (a, b):
+        assert np.isfinite(r["uh"]).all() and r["d"]["pmin"] > 0
+        assert abs(r["uh"][:, :, 0, 0].sum() - r["m0"]) <= 1e-12 * abs(r["m0"])
+    # the screen hands cells to KXRCF; TVB acts only where KXRCF flags: its
+    # decisions are candidates KXRCF flags, a subset of the plain method's
+    assert st["decision"] <= st["kxrcf"] and st["decision"] <= st["candidates"]
