@@ -42,7 +42,7 @@ EXPORTS = (
     "mhd2d_trainer_size", "mhd2d_trainer_clear", "mhd2d_trainer_get_data", "mhd2d_trainer_set_data",
     "mhd2d_trainer_grad", "mhd2d_trainer_fit",
     "mhd2d_checkpoint_save", "mhd2d_checkpoint_load", "mhd2d_ic_defaults", "mhd2d_calculate_az",
-    "mhd2d_calculate_l2error", "mhd2d_calculate_ez", "mhd2d_stream_triad",
+    "mhd2d_calculate_l2error", "mhd2d_calculate_ez", "mhd2d_stream_triad", "mhd2d_diagnostics_noalloc",
 )
 MLP_NPARAM = 8129        # W1[60][64], b1[64], W2[64][64], b2[64], w3[64], b3
 MLP_NIN = 60
@@ -137,6 +137,8 @@ def load_library(path: str = LIB_PATH):
         "mhd2d_calculate_l2error": ([P, C.c_double, dp], C.c_int),
         "mhd2d_calculate_ez": ([C.c_int64, dp, dp, dp, dp, dp], C.c_int),
         "mhd2d_stream_triad": ([C.c_int64, i32, i32, dp, dp], C.c_int),
+        "mhd2d_diagnostics_noalloc": ([i32, i32, C.c_double, C.c_double, C.c_double, C.c_double, P, P, P],
+                                      C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

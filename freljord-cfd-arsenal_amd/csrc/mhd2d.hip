@@ -21,6 +21,9 @@
 // This file: halo ring, reductions, initial conditions and the C ABI.
 // Kernels of the stage live in mhd2d_faces.hip, mhd2d_cell.hip and
 // mhd2d_limit.hip; shared declarations in mhd2d_internal.hpp.
+#include <atomic>
+#include <mutex>
+
 #include "mhd2d_internal.hpp"
 
 using namespace mhd;
@@ -190,9 +193,11 @@ __global__ __launch_bounds__(BS) void k_carry(const double* __restrict__ src, do
 }
 
 // calculate_umax.f90, calculate_pmin.f90 and per-block partial sums of
-// calculate_totaldiv.f90 (summed in a fixed order by k_div_final)
-template <bool MIX>
-__global__ __launch_bounds__(BS) void k_diag(SRd<MIX> U, Geo g, DevScal* sc, double* partial) {
+// calculate_totaldiv.f90 (summed in a fixed order by k_div_final).  V is the
+// state view: SRd<MIX> over a context's planes, or RefView over an array in
+// the reference layout (the allocation-free diagnostics)
+template <class V>
+__global__ __launch_bounds__(BS) void k_diag(V U, Geo g, DevScal* sc, double* partial) {
   const long long t = (long long)blockIdx.x * BS + threadIdx.x;
   double um = 0.0, pm = 1000000.0, dv = 0.0;
   if (t < (long long)g.Nx * g.Ny) {
@@ -245,7 +250,8 @@ __global__ __launch_bounds__(BS) void k_diag(SRd<MIX> U, Geo g, DevScal* sc, dou
   }
 }
 
-__global__ void k_div_final(DevScal* sc, const double* partial, long long nblocks, double hx1, double hy1) {
+__global__ void k_div_final(DevScal* sc, const double* partial, long long nblocks, double hx1, double hy1,
+                            double* out = nullptr) {
   __shared__ double s[BS];
   double a = 0.0;
   for (long long k = threadIdx.x; k < nblocks; k += BS) a += partial[k];
@@ -259,6 +265,7 @@ __global__ void k_div_final(DevScal* sc, const double* partial, long long nblock
     sc->pmin = dunkey(sc->pmin_key);
     if (!(isfinite(sc->totaldiv) && isfinite(sc->umax) && isfinite(sc->pmin))) sc->health |= MHD2D_FLAG_NONFINITE;
     if (sc->pmin < 0.0) sc->health |= MHD2D_FLAG_NEG_PRESSURE;
+    if (out) { out[0] = sc->umax; out[1] = sc->totaldiv; out[2] = sc->pmin; }
   }
 }
 
@@ -1236,7 +1243,7 @@ static int launch_diag(mhd2d_ctx* ctx) {
   hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, 1);
   with_mix(ctx, [&](auto Mc) -> int {
     constexpr bool MIX = decltype(Mc)::value;
-    hipLaunchKernelGGL((k_diag<MIX>), dim3(ctx->nblk_cells), dim3(BS), 0, ctx->stream, SRd<MIX>{ctx->S, g.plane}, g,
+    hipLaunchKernelGGL((k_diag<SRd<MIX>>), dim3(ctx->nblk_cells), dim3(BS), 0, ctx->stream, SRd<MIX>{ctx->S, g.plane}, g,
                        ctx->sc, ctx->partial);
     return 0;
   });
@@ -1568,3 +1575,84 @@ int mhd2d_profile_get(mhd2d_ctx* ctx, double* ms, int64_t* launches, int32_t ncl
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// calculate_umax / _totaldiv / _pmin without a context and without any
+// allocation.  Its two reductions need scratch memory (a DevScal of atomics
+// and one partial divergence sum per 256-cell block); it lives in the
+// file-scope device array below instead of a context's buffers.  The host
+// only ever passes the array's device address, looked up once per device.
+namespace {
+
+constexpr long long NOALLOC_MAX_CELLS = 4096LL * 4096;           // Nx * Ny
+constexpr long long NOALLOC_MAX_BLOCKS = NOALLOC_MAX_CELLS / BS;  // partial sums
+constexpr long long NOALLOC_SC_DOUBLES = 64;                      // the DevScal
+static_assert(sizeof(DevScal) <= NOALLOC_SC_DOUBLES * sizeof(double), "DevScal fits its scratch slot");
+// 512 KiB of partial sums + 512 B of scalars, held per device until exit
+static __device__ __align__(256) double scratch[NOALLOC_MAX_BLOCKS + NOALLOC_SC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::atomic<double*> noalloc_ptr[NOALLOC_MAX_DEVICES];
+std::mutex noalloc_mu;
+
+// the device address of `scratch` on the current device (hipGetSymbolAddress
+// once per device, with that device current); nullptr on failure
+double* noalloc_scratch(int dev) {
+  double* p = noalloc_ptr[dev].load(std::memory_order_acquire);
+  if (p) return p;
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  p = noalloc_ptr[dev].load(std::memory_order_relaxed);
+  if (p) return p;
+  void* a = nullptr;
+  if (hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)) != hipSuccess || !a) return nullptr;
+  p = static_cast<double*>(a);
+  noalloc_ptr[dev].store(p, std::memory_order_release);
+  return p;
+}
+
+// uh(0:Nx+1, 0:Ny+1, 10, 8) in device memory, column-major (com.txt:22):
+// plane p of PL() is the reference slot (mode b, variable n), b + 10 n, and
+// cell index c of cix() is OFF + i + (Nx + 2) j when the row pitch is Nx + 2
+struct RefView {
+  const double* b;
+  long long pl;   // (Nx + 2) (Ny + 2)
+  __device__ __forceinline__ double operator()(int p, long long c) const {
+    const int slot = p < 48 ? (p % 6) + 10 * (p / 6) : p == 48 ? 6 + 50 : p == 49 ? 8 + 50 : p == 50 ? 7 + 60 : 9 + 60;
+    return b[(long long)slot * pl + (c - OFF)];
+  }
+};
+
+}  // namespace
+
+extern "C" int mhd2d_diagnostics_noalloc(int32_t Nx, int32_t Ny, double xa, double xb, double ya, double yb,
+                                         const double* uh, double* out, void* stream) {
+  if (!uh || !out || Nx < 2 || Ny < 2 || !(xb > xa) || !(yb > ya)) return -1;
+  if ((long long)Nx * Ny > NOALLOC_MAX_CELLS) {
+    fprintf(stderr, "mhd2d_diagnostics_noalloc: %d x %d cells exceed its scratch maximum of %lld cells\n", Nx, Ny,
+            NOALLOC_MAX_CELLS);
+    return -3;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return -2;
+  double* base = noalloc_scratch(dev);
+  if (!base) return -2;
+  double* partial = base;
+  DevScal* sc = reinterpret_cast<DevScal*>(base + NOALLOC_MAX_BLOCKS);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Geo g{};
+  g.Nx = Nx;
+  g.Ny = Ny;
+  g.P = Nx + 2;   // cix(g, i, j) - OFF = i + (Nx + 2) j: the reference's (i, j) offset
+  g.plane = (long long)(Nx + 2) * (Ny + 2);
+  g.hx = (xb - xa) / Nx;
+  g.hy = (yb - ya) / Ny;
+  g.hx1 = 0.5 * g.hx;
+  g.hy1 = 0.5 * g.hy;
+  const unsigned nb = nblk((long long)Nx * Ny);
+  // the atomics' start values come from a kernel (no memset node under capture)
+  hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, st, sc, 1);
+  hipLaunchKernelGGL((k_diag<RefView>), dim3(nb), dim3(BS), 0, st, RefView{uh, g.plane}, g, sc, partial);
+  hipLaunchKernelGGL(k_div_final, dim3(1), dim3(BS), 0, st, sc, (const double*)partial, (long long)nb, g.hx1,
+                     g.hy1, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}

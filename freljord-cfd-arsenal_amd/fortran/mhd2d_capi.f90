@@ -177,6 +177,14 @@ module mhd2d_capi
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: jump
     end function
+    ! uh and out are device addresses (c_ptr); stream a hipStream_t or c_null_ptr
+    integer(c_int) function mhd2d_diagnostics_noalloc(Nx, Ny, xa, xb, ya, yb, uh, out, stream) &
+        bind(C, name='mhd2d_diagnostics_noalloc')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      integer(c_int32_t), value :: Nx, Ny
+      real(c_double), value :: xa, xb, ya, yb
+      type(c_ptr), value :: uh, out, stream
+    end function
     integer(c_int) function mhd2d_calculate_ez(n, u1, u2, B1, B2, Ez) bind(C, name='mhd2d_calculate_ez')
       import :: c_int, c_int64_t, c_double
       integer(c_int64_t), value :: n

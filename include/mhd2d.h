@@ -154,6 +154,24 @@ int mhd2d_diagnostics(mhd2d_ctx *ctx, double *umax, double *totaldiv, double *pm
    the face-jump half of the divergence report; div_free_Balsara.f90:43-58
    makes it zero up to rounding) */
 int mhd2d_face_jump(mhd2d_ctx *ctx, double *jump);
+/* mhd2d_diagnostics without a context and without allocating: no
+   counterpart in the reference; for callers that cannot allocate (code under
+   HIP graph capture, serving loops, harnesses that forbid hipMalloc).  uh is
+   the caller's device copy of uh(0:Nx+1, 0:Ny+1, 10, 8) in the reference
+   layout (column-major doubles), out the caller's device array of 3 doubles
+   that receives umax, totaldiv and pmin (calculate_umax/_totaldiv/_pmin.f90);
+   the domain gives hx, hy.  The kernels run on `stream` (a hipStream_t; NULL
+   is the null stream) and the call does not synchronise: out is ready when
+   the stream reaches that point.  The reductions' scratch (the per-block
+   partial sums and the atomics) is a static device array of 512.5 KiB that
+   each GPU the library runs on holds until the process ends, so Nx * Ny is
+   limited to 4096 * 4096 cells (-3 beyond, with a message on stderr).  All
+   calls on one GPU share that array: calls that overlap on one GPU (two
+   streams, or two host threads) race.  Returns 0, -1 for a null pointer, a
+   grid below 2 x 2 or an empty domain, -2 for a HIP error (the numerical
+   flags of mhd2d_diagnostics are not reported). */
+int mhd2d_diagnostics_noalloc(int32_t Nx, int32_t Ny, double xa, double xb, double ya, double yb, const double *uh,
+                              double *out, void *stream);
 /* calculate_Ez.f90:1-7: Ez[k] = u2[k]*B1[k] - u1[k]*B2[k] for k < n, on the
    host (the reference's pointwise helper; the vertex kernels evaluate the
    same function on the device).  Needs no context; -1 on a null pointer or
