@@ -168,7 +168,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 template <bool MIX>
 __global__ __launch_bounds__(256) void k_mlp_screen(SRd<MIX> U, int* __restrict__ screen,
                                                     float* __restrict__ prob, const MlpDev* __restrict__ W,
-                                                    Geo g, float l10Mh2, float l10h, float lo) {
+                                                    Geo g, float l10Mh2, float l10h, float lo, float hi) {
   if (step_off(g)) return;
   __shared__ __attribute__((aligned(16))) unsigned short sX[4][64 * XP];
   __shared__ float sZ[4][64];
@@ -251,11 +251,13 @@ __global__ __launch_bounds__(256) void k_mlp_screen(SRd<MIX> U, int* __restrict_
   if (!valid) return;
   const float p = __builtin_amdgcn_rcpf(1.0f + __expf(-sZ[wv][l]));
   prob[t] = p;
-  screen[t] = p >= lo ? 1 : 0;
+  // 2: the network decides the cell is troubled (TVB without KXRCF);
+  // 1: the confidence band, decided by KXRCF; 0: not troubled
+  screen[t] = p >= hi ? 2 : p >= lo ? 1 : 0;
 }
 
-// 1 where the screen marks interior cell (a, b) (0-based); outside the
-// domain the periodic image, or nothing for outflow boundaries
+// the screen mark (0, 1 or 2) of interior cell (a, b) (0-based); outside
+// the domain the periodic image, or 0 for outflow boundaries
 __device__ __forceinline__ int screen_at(const int* __restrict__ screen, const Geo& g, int a, int b) {
   if (a < 0) { if (g.bcL != 1) return 0; a += g.Nx; }
   if (a >= g.Nx) { if (g.bcR != 1) return 0; a -= g.Nx; }
@@ -288,12 +290,14 @@ __global__ void k_reset_nlist(DevScal* lst, const int* gate) {
   if (threadIdx.x == 0) lst->nlist = 0u;
 }
 
-// KXRCF_Detector on the candidates; flagged ones join the troubled-cell list
+// KXRCF_Detector on the candidates; flagged ones join the troubled-cell
+// list, and so do the cells the network itself decides (screen mark 2)
+// without the KXRCF evaluation
 template <bool MIX>
 __global__ __launch_bounds__(BS) void k_kxrcf_cands(SRd<MIX> U, int* __restrict__ trouble,
                                                     int* __restrict__ tmask, int* __restrict__ list,
-                                                    const int* __restrict__ blist, Geo g, double Ck, double hk,
-                                                    DevScal* sc, DevScal* lst) {
+                                                    const int* __restrict__ blist, const int* __restrict__ screen,
+                                                    Geo g, double Ck, double hk, DevScal* sc, DevScal* lst) {
   if (step_off(g)) return;
   const unsigned int nb = lst->nband;
   for (unsigned int k0 = blockIdx.x * BS; k0 < nb; k0 += gridDim.x * BS) {
@@ -303,7 +307,7 @@ __global__ __launch_bounds__(BS) void k_kxrcf_cands(SRd<MIX> U, int* __restrict_
     if (k < nb) {
       t = blist[k];
       const int i = 1 + t % g.Nx, j = 1 + t / g.Nx;
-      flag = kxrcf_cell(U, g, i, j, Ck, hk);
+      flag = screen[t] == 2 ? true : kxrcf_cell(U, g, i, j, Ck, hk);
       trouble[t] = flag ? 1 : 0;
       tmask[t] = 0;
     }
@@ -401,22 +405,37 @@ __global__ __launch_bounds__(BS) void k_dense_bwd(const float* __restrict__ dNex
   dZ[e] = s * gelu_d(Z[e]);
 }
 
-// dW[k][n] = sum_r In[r][k] dZ[r][n] (k < K), db[n] = sum_r dZ[r][n]; one
-// thread per (k, n) entry plus the bias row k = K
-__global__ __launch_bounds__(BS) void k_wgrad(const float* __restrict__ In, const long long* __restrict__ rows, int K,
-                                              const float* __restrict__ dZ, int N, float* __restrict__ dW,
-                                              float* __restrict__ db, int B) {
+// dW[k][n] = sum_r In[r][k] dZ[r][n] (k < K), db[n] = sum_r dZ[r][n], split
+// over the batch: block (x, c) takes WG_ROWS rows (chunk c) for BS entries
+// of the (K + 1) x N table (the bias row is k = K) and writes its partial
+// sums; k_wgrad_sum adds the chunks in a fixed order.  A batch of 8192 rows
+// then runs as 64 x 17 blocks instead of 17 blocks that each walk all rows.
+constexpr int WG_ROWS = 128;
+__global__ __launch_bounds__(BS) void k_wgrad_part(const float* __restrict__ In, const long long* __restrict__ rows,
+                                                   int K, const float* __restrict__ dZ, int N,
+                                                   float* __restrict__ part, int B) {
+  const int ne = (K + 1) * N;
   const int e = blockIdx.x * BS + threadIdx.x;
-  if (e >= (K + 1) * N) return;
+  if (e >= ne) return;
   const int k = e / N, n = e % N;
+  const int r0 = blockIdx.y * WG_ROWS, r1 = min(B, r0 + WG_ROWS);
   float s = 0.f;
   if (k < K) {
-    for (int r = 0; r < B; r++) s += In[(rows ? rows[r] : (long long)r) * K + k] * dZ[(long long)r * N + n];
-    dW[k * N + n] = s;
+    for (int r = r0; r < r1; r++) s += In[(rows ? rows[r] : (long long)r) * K + k] * dZ[(long long)r * N + n];
   } else {
-    for (int r = 0; r < B; r++) s += dZ[(long long)r * N + n];
-    db[n] = s;
+    for (int r = r0; r < r1; r++) s += dZ[(long long)r * N + n];
   }
+  part[(long long)blockIdx.y * ne + e] = s;
+}
+__global__ __launch_bounds__(BS) void k_wgrad_sum(const float* __restrict__ part, int nchunk, int K, int N,
+                                                  float* __restrict__ dW, float* __restrict__ db) {
+  const int ne = (K + 1) * N;
+  const int e = blockIdx.x * BS + threadIdx.x;
+  if (e >= ne) return;
+  float s = 0.f;
+  for (int c = 0; c < nchunk; c++) s += part[(long long)c * ne + e];
+  if (e < K * N) dW[e] = s;
+  else db[e - K * N] = s;
 }
 
 // fixed-order sum of a vector (one block)
@@ -454,7 +473,8 @@ __global__ __launch_bounds__(BS) void k_rows(long long* __restrict__ rows, int B
 }
 
 // indicator statistics over the interior cells (see mhd2d_indicator_stats);
-// the decision on a candidate is KXRCF's
+// the decision on a candidate is the network's where it marks the cell 2,
+// KXRCF's elsewhere
 __global__ __launch_bounds__(BS) void k_istats(const int* __restrict__ screen, const int* __restrict__ kx,
                                                const int* __restrict__ tm, Geo g,
                                                unsigned long long* __restrict__ out) {
@@ -462,7 +482,7 @@ __global__ __launch_bounds__(BS) void k_istats(const int* __restrict__ screen, c
   if (t >= (long long)g.Nx * g.Ny) return;
   const bool cand = candidate(screen, g, (int)(t % g.Nx), (int)(t / g.Nx));
   const bool k = kx[t] != 0, c = tm[t] != 0, lab = k && c;
-  const bool d = cand && k;
+  const bool d = cand && (k || screen[t] == 2);
   const bool hydro = (tm[t] & ~((1 << 5) | (1 << 6))) != 0;   // change bits of rho .. E, Bz
   const bool v[8] = {true, lab, k, d, cand, lab && !cand, cand && !lab, lab && !cand && hydro};
 #pragma unroll
@@ -518,7 +538,7 @@ static int launch_screen_only(mhd2d_ctx* ctx, const double* in) {
     hipLaunchKernelGGL((k_mlp_screen<MIX>), dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, ctx->stream,
                        SRd<MIX>{in, g.plane}, ctx->screen, ctx->prob, (const MlpDev*)ctx->mlpw, g,
                        (float)log10(ctx->cfg.M * g.hx * g.hy), (float)(0.5 * log10(sqrt(g.hx * g.hy))),
-                       (float)ctx->cfg.mlp_lo);
+                       (float)ctx->cfg.mlp_lo, (float)ctx->cfg.mlp_hi);
     LAUNCHCHK(ctx);
     return 0;
   });
@@ -543,7 +563,8 @@ int launch_candidates_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc) {
   int rc = with_mix(ctx, [&](auto Mc) -> int {
     constexpr bool MIX = decltype(Mc)::value;
     hipLaunchKernelGGL((k_kxrcf_cands<MIX>), dim3(nb), dim3(BS), 0, ctx->stream, SRd<MIX>{in, g.plane}, ctx->trouble,
-                       ctx->tmask, ctx->tlist, (const int*)ctx->blist, g, ctx->cfg.Ck, kx_hk(g), sc, ctx->sc);
+                       ctx->tmask, ctx->tlist, (const int*)ctx->blist, (const int*)ctx->screen, g, ctx->cfg.Ck,
+                       kx_hk(g), sc, ctx->sc);
     LAUNCHCHK(ctx);
     return 0;
   });
@@ -567,6 +588,7 @@ struct mhd2d_trainer {
   long long* rows;
   float *Z1, *A1, *Z2, *A2, *dz3, *loss, *dZ2, *dZ1, *lsum;
   float *P, *G, *Mo, *Vo;
+  float* wpart;                // weight-gradient partial sums, ceil(bmax / WG_ROWS) x 65 x 64
   char err[512];
 };
 
@@ -588,6 +610,15 @@ int tfail(mhd2d_trainer* tr, int code, const char* fmt, ...) {
 
 inline unsigned nb_of(long long n) { return (unsigned)((n + BS - 1) / BS); }
 
+void wgrad(mhd2d_trainer* tr, const float* In, const long long* rows, int K, const float* dZ, int N, float* dW,
+           float* db, int B) {
+  const int nchunk = (B + WG_ROWS - 1) / WG_ROWS;
+  hipLaunchKernelGGL(k_wgrad_part, dim3(nb_of((long long)(K + 1) * N), nchunk), dim3(BS), 0, tr->stream, In, rows, K,
+                     dZ, N, tr->wpart, B);
+  hipLaunchKernelGGL(k_wgrad_sum, dim3(nb_of((long long)(K + 1) * N)), dim3(BS), 0, tr->stream,
+                     (const float*)tr->wpart, nchunk, K, N, dW, db);
+}
+
 // forward + backward of one batch (rows of X given by rows[], or the B rows
 // from X itself when rows == nullptr); gradients into tr->G, loss into lsum
 int batch_grad(mhd2d_trainer* tr, const float* X, const float* Y, const long long* rows, int B, float pos_w) {
@@ -600,16 +631,13 @@ int batch_grad(mhd2d_trainer* tr, const float* X, const float* Y, const long lon
                      (const long long*)nullptr, 64, P + MLP_OFF_W2, P + MLP_OFF_B2, tr->Z2, tr->A2, B);
   hipLaunchKernelGGL(k_out_loss, dim3(nb_of(B)), dim3(BS), 0, st, (const float*)tr->A2, P + MLP_OFF_W3,
                      P + MLP_OFF_B3, Y, rows, pos_w, B, tr->dz3, tr->loss);
-  hipLaunchKernelGGL(k_wgrad, dim3(nb_of(65)), dim3(BS), 0, st, (const float*)tr->A2, (const long long*)nullptr, 64,
-                     (const float*)tr->dz3, 1, G + MLP_OFF_W3, G + MLP_OFF_B3, B);
+  wgrad(tr, tr->A2, nullptr, 64, tr->dz3, 1, G + MLP_OFF_W3, G + MLP_OFF_B3, B);
   hipLaunchKernelGGL(k_dense_bwd, dim3(nb_of((long long)B * 64)), dim3(BS), 0, st, (const float*)tr->dz3, 1,
                      P + MLP_OFF_W3, (const float*)tr->Z2, tr->dZ2, B);
-  hipLaunchKernelGGL(k_wgrad, dim3(nb_of(65 * 64)), dim3(BS), 0, st, (const float*)tr->A1, (const long long*)nullptr,
-                     64, (const float*)tr->dZ2, 64, G + MLP_OFF_W2, G + MLP_OFF_B2, B);
+  wgrad(tr, tr->A1, nullptr, 64, tr->dZ2, 64, G + MLP_OFF_W2, G + MLP_OFF_B2, B);
   hipLaunchKernelGGL(k_dense_bwd, dim3(nb_of((long long)B * 64)), dim3(BS), 0, st, (const float*)tr->dZ2, 64,
                      P + MLP_OFF_W2, (const float*)tr->Z1, tr->dZ1, B);
-  hipLaunchKernelGGL(k_wgrad, dim3(nb_of((MLP_NIN + 1) * 64)), dim3(BS), 0, st, X, rows, MLP_NIN,
-                     (const float*)tr->dZ1, 64, G + MLP_OFF_W1, G + MLP_OFF_B1, B);
+  wgrad(tr, X, rows, MLP_NIN, tr->dZ1, 64, G + MLP_OFF_W1, G + MLP_OFF_B1, B);
   hipLaunchKernelGGL(k_sum, dim3(1), dim3(BS), 0, st, (const float*)tr->loss, B, tr->lsum);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return tfail(tr, -3, "kernel launch: %s", hipGetErrorString(e));
@@ -670,7 +698,7 @@ int mhd2d_trainer_destroy(mhd2d_trainer* tr) {
   (void)hipSetDevice(tr->device);
   if (tr->stream) (void)hipStreamSynchronize(tr->stream);
   void* ps[] = {tr->X, tr->Y, tr->idx, tr->cnt, tr->rows, tr->Z1, tr->A1, tr->Z2, tr->A2, tr->dz3, tr->loss,
-                tr->dZ2, tr->dZ1, tr->lsum, tr->P, tr->G, tr->Mo, tr->Vo};
+                tr->dZ2, tr->dZ1, tr->lsum, tr->P, tr->G, tr->Mo, tr->Vo, tr->wpart};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (tr->stream) (void)hipStreamDestroy(tr->stream);
@@ -697,6 +725,7 @@ int mhd2d_trainer_create(int64_t capacity, int32_t max_batch, mhd2d_trainer** ou
       {(void**)&tr->dZ1, B * 64 * sizeof(float)}, {(void**)&tr->lsum, sizeof(float)},
       {(void**)&tr->P, MLP_NPARAM * sizeof(float)}, {(void**)&tr->G, MLP_NPARAM * sizeof(float)},
       {(void**)&tr->Mo, MLP_NPARAM * sizeof(float)}, {(void**)&tr->Vo, MLP_NPARAM * sizeof(float)},
+      {(void**)&tr->wpart, (B + WG_ROWS - 1) / WG_ROWS * 65 * 64 * sizeof(float)},
   };
   for (auto& a : al) {
     if (hipMalloc(a.p, a.n) != hipSuccess) {

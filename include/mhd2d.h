@@ -217,11 +217,13 @@ int mhd2d_stream_triad(int64_t n, int32_t unroll, int32_t reps, double *gbs, dou
 /* ---- learned troubled-cell indicator (no counterpart in the reference) ----
    A 60-64-64-1 MLP (GELU hidden layers, sigmoid output) over a cell's and its
    four neighbours' normalised modal coefficients gives p = P(the plain method
-   limits the cell).  With indicator = MHD2D_IND_MLP the limiter pass hands
-   the cells with p >= mlp_lo, dilated by one cell, to KXRCF_Detector and
-   skips the rest; inside mhd2d_step the network runs once per step, on the
-   first stage's limiter input, and its candidates serve all three stages.
-   mlp_hi is reserved.  Parameters are one FP32 vector of MHD2D_MLP_NPARAM
+   limits the cell).  With indicator = MHD2D_IND_MLP the network decides:
+   cells with p >= mlp_hi are limited (TVB) without KXRCF, the band of cells
+   with p >= mlp_lo, dilated by one cell, goes to KXRCF_Detector (the
+   fallback for low confidence), and the rest are not limited (mlp_hi >= 1
+   leaves every decision in the band to KXRCF).  Inside mhd2d_step the
+   network runs once per step, on the first stage's limiter input, and its
+   marks serve all three stages.  Parameters are one FP32 vector of MHD2D_MLP_NPARAM
    values: W1[60][64], b1[64], W2[64][64], b2[64], w3[64], b3 (W rows =
    inputs).  Inference runs in BF16 on the matrix cores with FP32
    accumulation. */
@@ -235,12 +237,13 @@ int mhd2d_mlp_probability(mhd2d_ctx *ctx, float *host_p);
    halos as they are: after mhd2d_rk_stage 1 that is the stage-1 limiter's
    input), out[8] = { cells, label
    (KXRCF flags AND TVB changes), KXRCF flags, decisions (candidates KXRCF
-   flags: TVB applied), candidates (cells handed to KXRCF), missed (label and
+   flags, or the network decides: TVB applied), candidates (cells handed to
+   KXRCF or decided), missed (label and
    not a candidate), over (candidate and not label), missed_hydro (missed
    cells whose TVB change includes a non-magnetic variable: changes to Bx, By
-   alone are overwritten by div_free_Balsara) }.  The screen can only skip
-   TVB applications, never add one: the decisions are a subset of the plain
-   method's. */
+   alone are overwritten by div_free_Balsara) }.  In the band the screen can
+   only skip TVB applications; only the cells it decides (p >= mlp_hi) may
+   be limited where the plain method would not limit them. */
 int mhd2d_indicator_stats(mhd2d_ctx *ctx, int64_t *out);
 
 /* FP32 training of the indicator on the GPU.  A trainer owns a device

@@ -128,8 +128,8 @@ def test_combined_4096(pkg):
     HLLD, mixed precision with the learned screen (KXRCF on its candidates),
     200 graph-replayed RK3 steps, against the plain FP64 method over the same
     200 steps: relative L2 <= 1e-5 in every field, div B <= 1e-12 in the
-    mixed form, mass conserved, finite state, and the screen's decisions a
-    subset of the plain method's"""
+    mixed form, mass conserved, finite state, and the screen's decisions no
+    more than the plain method's plus the cells the network decides alone"""
     N, dom, steps = 4096, (0.0, TWO_PI, 0.0, TWO_PI), 200
     z = np.load(pkg.WEIGHTS_PATH, allow_pickle=False)
     band = (float(z["mlp_lo"]), float(z["mlp_hi"]))
@@ -148,7 +148,8 @@ def test_combined_4096(pkg):
         uh = interior(s.download()[0])[:, :, :6, :].copy()
         d = s.diagnostics()
         res[name] = dict(uh=uh, d=d, t=s.get_time(), m0=m0, jump=s.face_jump(),
-                         stats=s.indicator_stats() if name == "mixed_nn" else None)
+                         stats=s.indicator_stats() if name == "mixed_nn" else None,
+                         decided=int((s.mlp_probability() >= band[1]).sum()) if name == "mixed_nn" else 0)
         s.close()
     a, b = res["mixed_nn"], res["fp64"]
     assert a["t"][2] == b["t"][2] == steps and abs(a["t"][0] - b["t"][0]) < 1e-6 * b["t"][0]
@@ -165,6 +166,7 @@ def test_combined_4096(pkg):
     for r in (a, b):
         assert np.isfinite(r["uh"]).all() and r["d"]["pmin"] > 0
         assert abs(r["uh"][:, :, 0, 0].sum() - r["m0"]) <= 1e-12 * abs(r["m0"])
-    # the screen hands cells to KXRCF; TVB acts only where KXRCF flags: its
-    # decisions are candidates KXRCF flags, a subset of the plain method's
-    assert st["decision"] <= st["kxrcf"] and st["decision"] <= st["candidates"]
+    # the screen hands its band to KXRCF and limits the cells it is sure of
+    # (p >= mlp_hi) itself: its decisions are the candidates KXRCF flags plus
+    # those cells
+    assert st["decision"] <= st["kxrcf"] + a["decided"] and st["decision"] <= st["candidates"]

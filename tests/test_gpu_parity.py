@@ -426,10 +426,10 @@ def test_mixed_storage_roundtrip(pkg):
 def test_mixed_shock_rotor(pkg):
     """the mixed form on a shock flow (the reference's rotor, init4.txt, with
     positivity and KXRCF-gated TVB): stays finite and divergence-free and
-    close to the plain form (measured 6e-6 in density after 20 steps at
-    128^2).  FP32 rounding can move a limiter decision at a discontinuity, so
-    the bar is looser than the smooth benchmark flow's 1e-5
-    (test_mixed_vs_fp64)."""
+    close to the plain form (measured 6.1e-6 in density after 20 steps at
+    128^2; the bar is about three times that).  FP32 rounding can move a
+    limiter decision at a discontinuity, so the bar is looser than the smooth
+    benchmark flow's 1e-5 (test_mixed_vs_fp64)."""
     N = 128
     out = {}
     for prec in (pkg.FP64, pkg.MIXED):
@@ -446,7 +446,7 @@ def test_mixed_shock_rotor(pkg):
           f"totaldiv {d32['totaldiv']:.2e}, pmin {d32['pmin']:.3e} / {d64['pmin']:.3e}")
     assert np.isfinite(u32).all()
     assert d32["totaldiv"] < 1e-12
-    assert err < 1e-4
+    assert err < 2e-5
 
 
 @pytest.mark.parametrize("ind", ["kxrcf", "none"])

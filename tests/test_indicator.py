@@ -127,6 +127,8 @@ def trained(pkg):
 
 
 MISS_BUDGET = {"rotor": 0.02, "riemann": 0.05}
+CAND_BUDGET = {"rotor": 0.25, "riemann": 0.9}
+ERR_BUDGET = {"rotor": 2.5e-3, "riemann": 2e-3}
 
 
 @pytest.mark.parametrize("kind", ["rotor", "riemann"])
@@ -155,19 +157,21 @@ def test_trained_indicator_heldout(pkg, trained, kind):
             s.step()
         runs[name] = s
     a, b = runs["mlp"].download()[0], runs["plain"].download()[0]
-    st = ind.limiter_input_stats(runs["plain"])
-    print(f"[indicator] {kind}: {st}")
-    # the classical detector is asked for a minority of the cells, and the
-    # learned decisions rarely differ from the plain method's where it matters
-    assert st["candidates"] <= 0.5 * st["cells"]
-    assert st["missed"] <= max(2, MISS_BUDGET[kind] * st["label"])
-    assert st["missed_hydro"] <= st["missed"]
     err = rel_l2(a[1:-1, 1:-1, :, 0], b[1:-1, 1:-1, :, 0])
     print(f"[indicator] {kind}: density rel L2 mlp vs plain after 20 steps: {err:.3e}")
+    st = ind.limiter_input_stats(runs["plain"])
+    print(f"[indicator] {kind}: {st}")
+    # the classical detector is asked for a part of the cells (most of a
+    # four-quadrant problem's cells lie next to one of its waves), and the
+    # learned decisions rarely differ from the plain method's where it matters
+    assert st["candidates"] <= CAND_BUDGET[kind] * st["cells"]
+    assert st["missed"] <= max(2, MISS_BUDGET[kind] * st["label"])
+    assert st["missed_hydro"] <= st["missed"]
     # shock-dominated flows: a single cell limited differently moves its
     # neighbourhood by O(jump) and the flow amplifies it; the 1e-5 budget is
-    # the smooth benchmark flow's (test_trained_indicator_ot)
-    assert err <= 0.1
+    # the smooth benchmark flow's (test_trained_indicator_ot).  Bars: about
+    # three times the measured values
+    assert err <= ERR_BUDGET[kind]
 
 
 def test_trained_indicator_ot(pkg, trained):
@@ -192,7 +196,11 @@ def test_trained_indicator_ot(pkg, trained):
     print(f"[indicator] ot: {st}")
     # the smooth flow: the network screens out nearly every cell
     assert st["candidates"] <= 0.05 * st["cells"] and st["missed"] <= max(2, 0.05 * st["label"])
-    assert st["decision"] <= st["kxrcf"]
+    # decisions: the candidates KXRCF flags, plus the cells the network
+    # decides by itself (p >= mlp_hi) on the same limiter input
+    decided = int((runs["plain"].mlp_probability() >= hi).sum())
+    print(f"[indicator] ot: cells decided by the network alone: {decided}")
+    assert st["decision"] <= st["kxrcf"] + decided
     err = rel_l2(a[1:-1, 1:-1, :, 0], b[1:-1, 1:-1, :, 0])
     print(f"[indicator] ot: density rel L2 mixed+mlp vs plain FP64 after 40 steps: {err:.3e}")
     assert err <= 1e-5
