@@ -637,6 +637,7 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
       {(void**)&ctx->tlist, (size_t)g.Nx * g.Ny * sizeof(int)},
       {(void**)&ctx->blist, (size_t)g.Nx * g.Ny * sizeof(int)},
       {(void**)&ctx->screen, (size_t)g.Nx * g.Ny * sizeof(int)},
+      {(void**)&ctx->pplist, (size_t)g.Nx * g.Ny * sizeof(int)},
       {(void**)&ctx->trec, (size_t)32 * g.Nx * g.Ny * sizeof(double)},
       {(void**)&ctx->prob, (size_t)g.Nx * g.Ny * sizeof(float)},
       {(void**)&ctx->sc, sizeof(DevScal)},
@@ -682,7 +683,8 @@ int mhd2d_destroy(mhd2d_ctx* ctx) {
   ctx->ev_pool.~vector();
   void* ps[] = {ctx->stage, ctx->S, ctx->U0, ctx->UI, ctx->UII, ctx->T, ctx->du, ctx->F, ctx->F0, ctx->FI, ctx->FII,
                 ctx->Ft, ctx->dEz, ctx->FX, ctx->FY, ctx->EZX, ctx->EZY, ctx->EZV, ctx->trouble, ctx->tmask,
-                ctx->tlist, ctx->blist, ctx->screen, ctx->trec, ctx->prob, ctx->mlpw, ctx->sc, ctx->partial, ctx->icp};
+                ctx->tlist, ctx->blist, ctx->screen, ctx->pplist, ctx->trec, ctx->prob, ctx->mlpw, ctx->sc,
+                ctx->partial, ctx->icp};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1350,10 +1352,13 @@ int step_body(mhd2d_ctx* ctx) {
   const bool pp = ctx->cfg.pp != 0;
   // one stage: fluxes of S; the CT update of the faces (Fa, Fb -> Fo); the
   // cell update (A, U0 -> out); the limiter pass (out -> lim, faces Fo).
-  // In the plain form, except with TVB on every cell, the limiter pass
-  // without TVB runs in the cell kernel's epilogue (the CT goes first so the
-  // new faces are there), and only the cells TVB changes are limited again.
-  const bool fused = ctx->cfg.indicator != MHD2D_IND_ALL && ctx->cfg.precision == MHD2D_FP64;
+  // Except with TVB on every cell, the limiter pass without TVB runs in the
+  // cell kernel's epilogue (the CT goes first so the new faces are there),
+  // and only the cells TVB changes are limited again; in the mixed form the
+  // epilogue leaves the cells its positivity bound cannot settle to a list
+  // kernel, which runs beside the detection.
+  const bool mixed = ctx->cfg.precision == MHD2D_MIXED;
+  const bool fused = ctx->cfg.indicator != MHD2D_IND_ALL;
   // The vertex EMFs (FP64 VALU-bound) and then the CT update run on a second
   // stream beside the face kernels (and, unfused, the cell kernel), forked
   // from and joined back into the context stream by events inside each stage
@@ -1393,7 +1398,16 @@ int step_body(mhd2d_ctx* ctx) {
       if ((r = on2(ct))) return r;
       if ((r = wait(ctx->ev_join, s2, s1))) return r;
       if ((r = launch_cell(ctx, k, ctx->S, A, ctx->U0, out, sc, 0.0, lim, Fo, pp))) return r;
-      return launch_limit_fixup(ctx, out, lim, Fo, pp, sc, k);
+      if (!mixed) return launch_limit_fixup(ctx, out, lim, Fo, pp, sc, k);
+      // detection and TVB on the second stream beside the positivity list;
+      // then the cells TVB changed are limited again (after the list, so
+      // that their TVB-limited value is the one that stays)
+      if ((r = wait(ctx->ev_cell, s1, s2))) return r;
+      if ((r = on2([&] { return launch_detection(ctx, out, sc, k); }))) return r;
+      if (two) HIPCHK(ctx, hipEventRecord(ctx->ev_det, s2));
+      if (pp && (r = launch_pp_list(ctx, out, lim, Fo))) return r;
+      if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_det, 0));
+      return launch_limit_relist(ctx, out, lim, Fo, pp);
     }
     // vertex EMFs and CT beside the memory-bound cell kernel
     if ((r = launch_fluxes(ctx, ctx->S, false, sc))) return r;

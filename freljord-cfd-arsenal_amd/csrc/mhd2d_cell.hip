@@ -35,16 +35,22 @@ namespace {
 // (positivity when PP, then the divergence-free reconstruction from the
 // stage's new faces Fn) to the stage value as stored, and writes the result
 // to lout; the cells TVB changes are limited again afterwards (k_limit_list).
-// This saves the limiter pass's re-read of the stage value and its faces;
-// in the mixed form the positivity epilogue needs 254 VGPRs, which halves the
-// kernel's waves per SIMD and costs more than the pass it saves (measured)
+// This saves the limiter pass's re-read of the stage value and its faces.
+// In the mixed form the epilogue is lean: a full positivity epilogue needs
+// 254 VGPRs, which halves the kernel's waves per SIMD and costs more than
+// the pass it saves (measured).  Instead the epilogue accumulates, plane by
+// plane as it stores them, the sums of the positivity bound (pp_settled);
+// the cells the bound settles (nearly all of them) get the stage value with
+// the divergence-free B planes as their limited value, the others are
+// appended to pplist and limited afterwards by a list kernel (k_pp_list).
 template <int MODE, typename T, int CBS, bool SQ = false, bool MIX = (sizeof(T) == 4), bool LIM = false,
           bool PP = false>
-__global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ FX,
+__global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 1) ? 4 : 1) void k_cell(SRd<MIX> S, const T* __restrict__ FX,
                                               const T* __restrict__ FY, SRd<MIX> A,
                                               SRd<MIX> U0, SWr<MIX> out, SWr<MIX> lout,
                                               const double* __restrict__ Fn, Geo g,
-                                              const DevScal* __restrict__ sc, double dt_arg) {
+                                              const DevScal* __restrict__ sc, double dt_arg,
+                                              int* __restrict__ pplist = nullptr, DevScal* lst = nullptr) {
   if (step_off(g)) return;
   // The mixed form keeps the first NR planes of the cell in registers and the
   // other 40 in LDS: 160 B per lane, four 256-lane blocks (16 waves) per CU
@@ -215,7 +221,11 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
     }
   }
   // epilogue: mass inverse + RK combination, in double
-  double m[LIM ? NPL : 1];   // LIM: the stage value as stored (FP32 planes rounded)
+  constexpr bool LEAN = LIM && MIX;   // the mixed form's lean limiter epilogue
+  double m[LIM && !LEAN ? NPL : 1];   // LIM: the stage value as stored (FP32 planes rounded)
+  double pa[LEAN ? 8 : 1], pav[LEAN ? 8 : 1];   // LEAN: positivity bound sums and the averages
+#pragma unroll
+  for (int n = 0; n < (LEAN ? 8 : 1); n++) { pa[n] = 0.0; pav[n] = 0.0; }
   auto plane = [&](auto P) {
     const int p = P;
     const bool avg = p < 48 && (p % 6) == 0;
@@ -228,7 +238,15 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
     else if (MODE == 2) r = (3.0 / 4.0) * U0(p, c) + (1.0 / 4.0) * (A(p, c) + dt * du);
     else r = (1.0 / 3.0) * U0(p, c) + (2.0 / 3.0) * (A(p, c) + dt * du);
     out.set(p, c, r);
-    if constexpr (LIM) m[p] = (MIX && !pl_f64(p)) ? (double)(float)r : r;
+    if constexpr (LIM && !LEAN) m[p] = (MIX && !pl_f64(p)) ? (double)(float)r : r;
+    if constexpr (LEAN) {
+      const double v = pl_f64(p) ? r : (double)(float)r;   // the value as stored
+      if (!is_bplane(p)) lout.set(p, c, v);                  // the limited value of a settled cell
+      else if (!interior) lout.set(p, c, 0.0);               // div_free_Balsara.f90:13 zeroes the halo B
+      const int n = p < 48 ? p / 6 : (p < 50 ? 5 : 6);
+      if (avg) pav[n] = v;
+      else pa[n] += fabs(v) * pp_pmax(b);                    // b in increasing order for every n
+    }
   };
   // the mixed storage map needs the plane as a compile-time constant (a
   // partially unrolled loop would index acc[] dynamically, i.e. in scratch)
@@ -238,7 +256,15 @@ __global__ __launch_bounds__(CBS) void k_cell(SRd<MIX> S, const T* __restrict__ 
 #pragma unroll
     for (int p = 0; p < NPL; p++) plane(p);
   }
-  if constexpr (LIM) {
+  if constexpr (LEAN) {
+    bool need = false;
+    if (interior) {
+      const bool settled = !PP || pp_settled_sums(pa, pav, 1e-13);
+      if (settled) divfree_apply([&](int q, double v) { lout.set(q, c, v); }, Fn, g, i, j);
+      need = !settled;
+    }
+    if constexpr (PP) pp_append(need, interior ? (j - 1) * g.Nx + (i - 1) : 0, pplist, lst);
+  } else if constexpr (LIM) {
     // the limiter pass of k_limit<false, PP, true> on the stored stage value
     if (interior) {
       if (PP) pp_cell(m);
@@ -351,6 +377,11 @@ __global__ __launch_bounds__(BS) void k_ct(const T* __restrict__ EZX, const T* _
   }
 }
 
+__global__ void k_reset_npp(DevScal* lst, const int* gate) {
+  if (step_off(gate)) return;
+  if (threadIdx.x == 0) lst->npp = 0u;
+}
+
 // the FP32 planes of the mixed storage: combined in double, stored in float
 __global__ __launch_bounds__(BS) void k_axpy_f(float* __restrict__ out, const float* __restrict__ X, double a,
                                                const float* __restrict__ Y, double b, const float* __restrict__ Z,
@@ -384,7 +415,24 @@ void launch_cell_t(mhd2d_ctx* ctx, const double* S, const double* A, const doubl
   const Geo& g = ctx->g;
   const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
   const long long pl = g.plane;
-  if constexpr (LIM) {   // plain form only
+  if constexpr (LIM) {
+    if (ctx->cfg.precision == MHD2D_MIXED) {
+      // the lean epilogue's positivity list starts empty
+      hipLaunchKernelGGL(k_reset_npp, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, g.gate);
+      if (g.hx1 == g.hy1)
+        hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, true, true, true, PP>),
+                           dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)), dim3(CBS_F32), 0, ctx->stream,
+                           SRd<true>{S, pl}, (const float*)ctx->FX, (const float*)ctx->FY, SRd<true>{A, pl},
+                           SRd<true>{U0, pl}, SWr<true>{out, pl}, SWr<true>{lout, pl}, Fn, g, sc, dt, ctx->pplist,
+                           ctx->sc);
+      else
+        hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, false, true, true, PP>),
+                           dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)), dim3(CBS_F32), 0, ctx->stream,
+                           SRd<true>{S, pl}, (const float*)ctx->FX, (const float*)ctx->FY, SRd<true>{A, pl},
+                           SRd<true>{U0, pl}, SWr<true>{out, pl}, SWr<true>{lout, pl}, Fn, g, sc, dt, ctx->pplist,
+                           ctx->sc);
+      return;
+    }
     hipLaunchKernelGGL((k_cell<MODE, double, CBS_F64, false, false, true, PP>),
                        dim3((unsigned)((n + CBS_F64 - 1) / CBS_F64)), dim3(CBS_F64), 0, ctx->stream, SRd<false>{S, pl},
                        (const double*)ctx->FX, (const double*)ctx->FY, SRd<false>{A, pl}, SRd<false>{U0, pl},
@@ -392,15 +440,15 @@ void launch_cell_t(mhd2d_ctx* ctx, const double* S, const double* A, const doubl
     return;
   }
   if (ctx->cfg.precision == MHD2D_MIXED && g.hx1 == g.hy1)
-    hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, true, true, LIM, PP>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
+    hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, true, true, false, false>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
                        dim3(CBS_F32), 0, ctx->stream, SRd<true>{S, pl}, (const float*)ctx->FX, (const float*)ctx->FY,
                        SRd<true>{A, pl}, SRd<true>{U0, pl}, SWr<true>{out, pl}, SWr<true>{lout, pl}, Fn, g, sc, dt);
   else if (ctx->cfg.precision == MHD2D_MIXED)
-    hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, false, true, LIM, PP>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
+    hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, false, true, false, false>), dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)),
                        dim3(CBS_F32), 0, ctx->stream, SRd<true>{S, pl}, (const float*)ctx->FX, (const float*)ctx->FY,
                        SRd<true>{A, pl}, SRd<true>{U0, pl}, SWr<true>{out, pl}, SWr<true>{lout, pl}, Fn, g, sc, dt);
   else
-    hipLaunchKernelGGL((k_cell<MODE, double, CBS_F64, false, false, LIM, PP>), dim3((unsigned)((n + CBS_F64 - 1) / CBS_F64)),
+    hipLaunchKernelGGL((k_cell<MODE, double, CBS_F64, false, false, false, false>), dim3((unsigned)((n + CBS_F64 - 1) / CBS_F64)),
                        dim3(CBS_F64), 0, ctx->stream, SRd<false>{S, pl}, (const double*)ctx->FX,
                        (const double*)ctx->FY, SRd<false>{A, pl}, SRd<false>{U0, pl}, SWr<false>{out, pl},
                        SWr<false>{lout, pl}, Fn, g, sc, dt);
@@ -419,8 +467,7 @@ void launch_cell_mode(mhd2d_ctx* ctx, int mode, const double* S, const double* A
 
 int launch_cell(mhd2d_ctx* ctx, int mode, const double* S, const double* A, const double* U0, double* out,
                 const DevScal* sc, double dt, double* lout, const double* Fn, bool pp) {
-  if (lout && (mode == 0 || ctx->cfg.precision != MHD2D_FP64))
-    return fail(ctx, -1, "launch_cell: the fused limiter is for the plain form's RK stages");
+  if (lout && mode == 0) return fail(ctx, -1, "launch_cell: the fused limiter is for the RK stages");
   Timed tm(ctx, KC_CELL);
   if (!lout) launch_cell_mode<false, false>(ctx, mode, S, A, U0, out, nullptr, nullptr, sc, dt);
   else if (pp) launch_cell_mode<true, true>(ctx, mode, S, A, U0, out, lout, Fn, sc, dt);

@@ -35,6 +35,7 @@ struct DevScal {
   long long nflag;
   unsigned int nlist;   // length of the troubled-cell list of the current limiter pass
   unsigned int nband;   // length of the indicator's candidate list (cells handed to KXRCF)
+  unsigned int npp;     // length of the list of cells the positivity limiter must look at (mixed form)
   int active;
   int health;           // MHD2D_FLAG_* bits, sticky until mhd2d_set_time
 };
@@ -165,6 +166,7 @@ struct mhd2d_ctx {
   int* tlist;          // interior-cell indices the detector flagged (length sc->nlist)
   int* blist;          // interior-cell indices the indicator hands to KXRCF (sc->nband)
   int* screen;         // indicator: 1 where p >= mlp_lo (before the one-cell dilation)
+  int* pplist;         // mixed form: interior cells whose positivity bound does not settle them (sc->npp)
   double* trec;        // modified edge values of changed cells: 32 planes over the interior cells
   void* mlpw;          // indicator MLP weights in MFMA operand order (mhd2d_mlp.hip)
   float* prob;         // indicator MLP probability per interior cell
@@ -291,6 +293,9 @@ int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F,
 // limited again from `in` into out.  launch_limit_fixup = both.
 int launch_detection(mhd2d_ctx* ctx, const double* in, DevScal* sc, int stage);                    // mhd2d_limit.hip
 int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp);  // mhd2d_limit.hip
+// the cells the mixed cell kernel's epilogue left to the positivity limiter
+// (the list of its bound test), limited from `in` into out
+int launch_pp_list(mhd2d_ctx* ctx, const double* in, double* out, const double* F);                 // mhd2d_limit.hip
 int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, DevScal* sc,
                        int stage);                                                                           // mhd2d_limit.hip
 

@@ -86,6 +86,22 @@ __global__ __launch_bounds__(BS) void k_limit_list(SRd<MIX> in, SWr<MIX> out, co
   }
 }
 
+// the cells the mixed cell kernel's epilogue could not settle with the
+// positivity bound: the full limiter pass without TVB (positivity, then the
+// divergence-free reconstruction) from the unlimited stage value; the cells
+// TVB changes are limited again afterwards (k_limit_list)
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_pp_list(SRd<MIX> in, SWr<MIX> out, const double* __restrict__ F,
+                                                const int* __restrict__ list, Geo g,
+                                                const DevScal* __restrict__ lst) {
+  if (step_off(g)) return;
+  const unsigned int n = lst->npp;
+  for (unsigned int k = blockIdx.x * BS + threadIdx.x; k < n; k += gridDim.x * BS) {
+    const int tc = list[k];
+    limit_cell<false, true, true, MIX>(in, out, F, nullptr, nullptr, g, 1 + tc % g.Nx, 1 + tc / g.Nx);
+  }
+}
+
 template <bool MIX>
 __global__ __launch_bounds__(BS) void k_kxrcf(SRd<MIX> U, int* __restrict__ trouble, Geo g,
                                               double Ck, double hk, DevScal* sc) {
@@ -372,6 +388,17 @@ int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const dou
     LAUNCHCHK(ctx);
     return 0;
   });
+}
+
+int launch_pp_list(mhd2d_ctx* ctx, const double* in, double* out, const double* F) {
+  if (ctx->cfg.precision != MHD2D_MIXED) return 0;
+  const Geo& g = ctx->g;
+  Timed tm(ctx, KC_LIMIT);
+  const unsigned nb = std::min<unsigned>(nblk((long long)g.Nx * g.Ny), 1024u);
+  hipLaunchKernelGGL((k_pp_list<true>), dim3(nb), dim3(BS), 0, ctx->stream, SRd<true>{in, g.plane},
+                     SWr<true>{out, g.plane}, F, (const int*)ctx->pplist, g, (const DevScal*)ctx->sc);
+  LAUNCHCHK(ctx);
+  return 0;
 }
 
 int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, DevScal* sc,

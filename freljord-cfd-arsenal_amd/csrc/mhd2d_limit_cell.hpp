@@ -30,36 +30,47 @@ __device__ __forceinline__ double glleval(const double* m, int n, int i1, int j1
 // (by more than 1e-10 of its terms, far above their rounding), no point of
 // either pass below fires, eta stays 1 and the final scaling by 1 leaves the
 // modes as they are: returning early gives the same bits.
-__device__ __forceinline__ bool pp_settled(const double m[NPL], double eps) {
-  constexpr double pmax[10] = {1.0, 1.0, 1.0, 2.0 / 3.0, 1.0, 2.0 / 3.0, 0.4, 2.0 / 3.0, 2.0 / 3.0, 0.4};
+__host__ __device__ constexpr double pp_pmax(int b) {
+  return b == 0 || b == 1 || b == 2 || b == 4 ? 1.0 : b == 6 || b == 9 ? 0.4 : 2.0 / 3.0;
+}
+// the test from the sums a[n] = sum_{b >= 1} |m_b| max|phi_b| (summed over b
+// in increasing order) and the averages avg[n]
+__device__ __forceinline__ bool pp_settled_sums(const double a[8], const double avg[8], double eps) {
   double d[8];
 #pragma unroll
-  for (int n = 0; n < 8; n++) {
-    double a = 0.0;
-#pragma unroll
-    for (int b = 1; b < 10; b++) {
-      const int p = PL(b, n);
-      if (p >= 0) a += fabs(m[p]) * pmax[b];
-    }
-    d[n] = a * (1.0 + 1e-12) + 1e-12 * (fabs(m[PL(0, n)]) + a);
-  }
-  const double rbar = m[PL(0, 0)];
+  for (int n = 0; n < 8; n++) d[n] = a[n] * (1.0 + 1e-12) + 1e-12 * (fabs(avg[n]) + a[n]);
+  const double rbar = avg[0];
   const double rlo = rbar - d[0];
   if (!(d[0] < fabs(rbar - eps)) || !(rlo > 0.0)) return false;
   double kin = 0.0, mag = 0.0;
 #pragma unroll
   for (int n = 1; n < 4; n++) {
-    const double a = fabs(m[PL(0, n)]) + d[n];
-    kin += a * a;
+    const double v = fabs(avg[n]) + d[n];
+    kin += v * v;
   }
 #pragma unroll
   for (int n = 5; n < 8; n++) {
-    const double a = fabs(m[PL(0, n)]) + d[n];
-    mag += a * a;
+    const double v = fabs(avg[n]) + d[n];
+    mag += v * v;
   }
   const double k = 0.5 * kin / rlo, b2 = 0.5 * mag;
-  const double plo = (m[PL(0, 4)] - d[4]) - k - b2;
-  return plo > 1e-10 * (fabs(m[PL(0, 4)]) + d[4] + k + b2);
+  const double plo = (avg[4] - d[4]) - k - b2;
+  return plo > 1e-10 * (fabs(avg[4]) + d[4] + k + b2);
+}
+__device__ __forceinline__ bool pp_settled(const double m[NPL], double eps) {
+  double a[8], avg[8];
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    double s = 0.0;
+#pragma unroll
+    for (int b = 1; b < 10; b++) {
+      const int p = PL(b, n);
+      if (p >= 0) s += fabs(m[p]) * pp_pmax(b);
+    }
+    a[n] = s;
+    avg[n] = m[PL(0, n)];
+  }
+  return pp_settled_sums(a, avg, eps);
 }
 
 __device__ __forceinline__ void pp_cell(double m[NPL]) {
@@ -169,8 +180,10 @@ __device__ __forceinline__ void pp_cell(double m[NPL]) {
     }
 }
 
-// div_free_Balsara.f90:15-78: in-cell B from the four faces
-__device__ __forceinline__ void divfree_cell(double m[NPL], const double* __restrict__ F, const Geo& g, int i, int j) {
+// div_free_Balsara.f90:15-78: in-cell B from the four faces; set(p, v)
+// receives plane p of the reconstruction
+template <class Set>
+__device__ __forceinline__ void divfree_apply(Set&& set, const double* __restrict__ F, const Geo& g, int i, int j) {
   const long long pl = g.plane;
   const long long cR = cix(g, i, j), cL = cix(g, i - 1, j), cD = cix(g, i, j - 1);
   const double a0R = F[0 * pl + cR], a1R = F[1 * pl + cR], a2R = F[2 * pl + cR];
@@ -178,22 +191,38 @@ __device__ __forceinline__ void divfree_cell(double m[NPL], const double* __rest
   const double b0U = F[3 * pl + cR], b1U = F[4 * pl + cR], b2U = F[5 * pl + cR];
   const double b0D = F[3 * pl + cD], b1D = F[4 * pl + cD], b2D = F[5 * pl + cD];
   const double rxy = g.hx / g.hy, ryx = g.hy / g.hx;
-  m[PL(0, 5)] = 0.5 * (a0R + a0L) + (1.0 / 6.0) * rxy * (b1U - b1D);
-  m[PL(1, 5)] = 0.5 * (a0R - a0L) + (1.0 / 15.0) * rxy * (b2U - b2D);
-  m[PL(2, 5)] = 0.5 * (a1R + a1L);
-  m[PL(3, 5)] = -(1.0 / 4.0) * rxy * (b1U - b1D);
-  m[PL(4, 5)] = 0.5 * (a1R - a1L);
-  m[PL(5, 5)] = 0.5 * (a2R + a2L);
-  m[PL(6, 5)] = -(1.0 / 6.0) * rxy * (b2U - b2D);
-  m[PL(8, 5)] = 0.5 * (a2R - a2L);
-  m[PL(0, 6)] = 0.5 * (b0U + b0D) + (1.0 / 6.0) * ryx * (a1R - a1L);
-  m[PL(2, 6)] = 0.5 * (b0U - b0D) + (1.0 / 15.0) * ryx * (a2R - a2L);
-  m[PL(1, 6)] = 0.5 * (b1U + b1D);
-  m[PL(5, 6)] = -(1.0 / 4.0) * ryx * (a1R - a1L);
-  m[PL(4, 6)] = 0.5 * (b1U - b1D);
-  m[PL(3, 6)] = 0.5 * (b2U + b2D);
-  m[PL(9, 6)] = -(1.0 / 6.0) * ryx * (a2R - a2L);
-  m[PL(7, 6)] = 0.5 * (b2U - b2D);
+  set(PL(0, 5), 0.5 * (a0R + a0L) + (1.0 / 6.0) * rxy * (b1U - b1D));
+  set(PL(1, 5), 0.5 * (a0R - a0L) + (1.0 / 15.0) * rxy * (b2U - b2D));
+  set(PL(2, 5), 0.5 * (a1R + a1L));
+  set(PL(3, 5), -(1.0 / 4.0) * rxy * (b1U - b1D));
+  set(PL(4, 5), 0.5 * (a1R - a1L));
+  set(PL(5, 5), 0.5 * (a2R + a2L));
+  set(PL(6, 5), -(1.0 / 6.0) * rxy * (b2U - b2D));
+  set(PL(8, 5), 0.5 * (a2R - a2L));
+  set(PL(0, 6), 0.5 * (b0U + b0D) + (1.0 / 6.0) * ryx * (a1R - a1L));
+  set(PL(2, 6), 0.5 * (b0U - b0D) + (1.0 / 15.0) * ryx * (a2R - a2L));
+  set(PL(1, 6), 0.5 * (b1U + b1D));
+  set(PL(5, 6), -(1.0 / 4.0) * ryx * (a1R - a1L));
+  set(PL(4, 6), 0.5 * (b1U - b1D));
+  set(PL(3, 6), 0.5 * (b2U + b2D));
+  set(PL(9, 6), -(1.0 / 6.0) * ryx * (a2R - a2L));
+  set(PL(7, 6), 0.5 * (b2U - b2D));
+}
+__device__ __forceinline__ void divfree_cell(double m[NPL], const double* __restrict__ F, const Geo& g, int i, int j) {
+  divfree_apply([&](int p, double v) { m[p] = v; }, F, g, i, j);
+}
+
+// append the flagged lanes' interior-cell ids to the positivity list (one
+// atomic per wave)
+__device__ __forceinline__ void pp_append(bool flag, int t, int* __restrict__ list, DevScal* lst) {
+  const unsigned long long b = __ballot(flag);
+  if (b) {
+    const int lane = __lane_id(), leader = __ffsll((long long)b) - 1;
+    unsigned int base = 0u;
+    if (lane == leader) base = atomicAdd(&lst->npp, (unsigned int)__popcll(b));
+    base = (unsigned int)__shfl((int)base, leader);
+    if (flag) list[base + (unsigned int)__popcll(b & ((1ull << lane) - 1ull))] = t;
+  }
 }
 
 __device__ __forceinline__ bool is_bplane(int p) { return (p >= 30 && p < 42) || p >= 48; }
