@@ -162,7 +162,7 @@ def cpu_baseline(N=512, steps=1):
 
 
 def counters_run(pkg, N, dom):
-    """Calibration dispatch plus two steps of each benchmarked form (plain
+    """Calibration dispatches plus two steps of each benchmarked form (plain
     FP64, mixed, mixed + learned indicator), for rocprofv3 --pmc.
 
     The calibration kernel is k_axpy of the per-subroutine stage update
@@ -178,6 +178,12 @@ def counters_run(pkg, N, dom):
     n = 52 * (N + 2) * P
     print(f"[counters] k_axpy over {n} doubles: reads {16 * n} B, writes {8 * n} B", file=sys.stderr, flush=True)
     s.close()
+    # a second calibration kernel: the triad over three arrays of 2^28
+    # doubles (one warm-up and one timed launch, each reading 4 GiB and
+    # writing 2 GiB)
+    pkg.stream_triad(1 << 28, 1, 1)
+    print(f"[counters] k_triad<1> over {1 << 28} doubles: reads {16 << 28} B, writes {8 << 28} B", file=sys.stderr,
+          flush=True)
     z = np.load(pkg.WEIGHTS_PATH, allow_pickle=False)
     params, band = pkg.load_indicator_weights(), (float(z["mlp_lo"]), float(z["mlp_hi"]))
     for name, prec, ind in (("fp64", pkg.FP64, pkg.IND_KXRCF), ("mixed", pkg.MIXED, pkg.IND_KXRCF),
