@@ -1361,8 +1361,7 @@ int step_body(mhd2d_ctx* ctx) {
   const bool fused = ctx->cfg.indicator != MHD2D_IND_ALL;
   // The vertex EMFs (FP64 VALU-bound) and then the CT update run on a second
   // stream beside the face kernels (and, unfused, the cell kernel), forked
-  // from and joined back into the context stream by events inside each stage
-  // (graph capture records the same fork and join as parallel branches).
+  // from and joined back into the context stream by events inside each stage.
   // With the per-kernel event timing on, everything stays on one stream so
   // that each kernel is timed alone.  Graph capture also stays on one stream:
   // with this ROCm, a captured graph with parallel branches keeps the buffers
