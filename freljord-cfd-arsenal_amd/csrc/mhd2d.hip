@@ -1356,7 +1356,8 @@ int step_body(mhd2d_ctx* ctx) {
   // cell kernel's epilogue (the CT goes first so the new faces are there),
   // and only the cells TVB changes are limited again; in the mixed form the
   // epilogue leaves the cells its positivity bound cannot settle to a list
-  // kernel, which runs beside the detection.
+  // kernel, which runs beside the detection, and the limited B planes to a
+  // pass after the CT that runs beside the cell kernel.
   const bool mixed = ctx->cfg.precision == MHD2D_MIXED;
   const bool fused = ctx->cfg.indicator != MHD2D_IND_ALL;
   // The vertex EMFs (FP64 VALU-bound) and then the CT update run on a second
@@ -1387,7 +1388,7 @@ int step_body(mhd2d_ctx* ctx) {
     auto ct = [&] { return launch_ct(ctx, k, k == 1 ? Fa : nullptr, k == 1 ? nullptr : Fa, Fb, Fo, sc, 0.0); };
     auto vertex = [&] { return launch_vertex(ctx, ctx->S, sc); };
     const hipStream_t s1 = ctx->stream, s2 = ctx->stream2;
-    if (fused) {
+    if (fused && !mixed) {
       // vertex EMFs beside the face kernels; the cell kernel's limiter
       // epilogue reads the new faces, so it follows the CT
       if ((r = wait(ctx->ev_fork, s1, s2))) return r;
@@ -1397,13 +1398,27 @@ int step_body(mhd2d_ctx* ctx) {
       if ((r = on2(ct))) return r;
       if ((r = wait(ctx->ev_join, s2, s1))) return r;
       if ((r = launch_cell(ctx, k, ctx->S, A, ctx->U0, out, sc, 0.0, lim, Fo, pp))) return r;
-      if (!mixed) return launch_limit_fixup(ctx, out, lim, Fo, pp, sc, k);
+      return launch_limit_fixup(ctx, out, lim, Fo, pp, sc, k);
+    }
+    if (fused) {
+      // mixed form: vertex EMFs beside the face kernels, then the CT update
+      // and the divergence-free B planes of the limited value beside the
+      // cell kernel (its lean epilogue writes the other planes and needs no
+      // faces)
+      if ((r = wait(ctx->ev_fork, s1, s2))) return r;
+      if ((r = on2(vertex))) return r;
+      if ((r = launch_fluxes(ctx, ctx->S, false, sc))) return r;
+      if ((r = wait(ctx->ev_faces, s1, s2))) return r;
+      if ((r = on2(ct)) || (r = on2([&] { return launch_divfree(ctx, lim, Fo); }))) return r;
+      if (two) HIPCHK(ctx, hipEventRecord(ctx->ev_join, s2));       // new faces and limited B done
+      if ((r = launch_cell(ctx, k, ctx->S, A, ctx->U0, out, sc, 0.0, lim, Fo, pp))) return r;
       // detection and TVB on the second stream beside the positivity list;
       // then the cells TVB changed are limited again (after the list, so
       // that their TVB-limited value is the one that stays)
       if ((r = wait(ctx->ev_cell, s1, s2))) return r;
       if ((r = on2([&] { return launch_detection(ctx, out, sc, k); }))) return r;
       if (two) HIPCHK(ctx, hipEventRecord(ctx->ev_det, s2));
+      if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_join, 0));
       if (pp && (r = launch_pp_list(ctx, out, lim, Fo))) return r;
       if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_det, 0));
       return launch_limit_relist(ctx, out, lim, Fo, pp);

@@ -39,13 +39,16 @@ namespace {
 // In the mixed form the epilogue is lean: a full positivity epilogue needs
 // 254 VGPRs, which halves the kernel's waves per SIMD and costs more than
 // the pass it saves (measured).  Instead the epilogue accumulates, plane by
-// plane as it stores them, the sums of the positivity bound (pp_settled);
-// the cells the bound settles (nearly all of them) get the stage value with
-// the divergence-free B planes as their limited value, the others are
-// appended to pplist and limited afterwards by a list kernel (k_pp_list).
+// plane as it stores them, the sums of the positivity bound (pp_settled) and
+// writes the planes other than Bx, By of the stage value as the limited
+// value; the cells the bound cannot settle are appended to pplist and limited
+// afterwards by a list kernel (k_pp_list).  The limited B planes come from
+// the faces alone (div_free_Balsara), so a separate pass (k_divfree) writes
+// them after the CT update, beside this kernel, and the kernel does not wait
+// for the CT.
 template <int MODE, typename T, int CBS, bool SQ = false, bool MIX = (sizeof(T) == 4), bool LIM = false,
           bool PP = false>
-__global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 1) ? 4 : 1) void k_cell(SRd<MIX> S, const T* __restrict__ FX,
+__global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(SRd<MIX> S, const T* __restrict__ FX,
                                               const T* __restrict__ FY, SRd<MIX> A,
                                               SRd<MIX> U0, SWr<MIX> out, SWr<MIX> lout,
                                               const double* __restrict__ Fn, Geo g,
@@ -242,7 +245,6 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 1) ? 4 : 1) void k_cell(
     if constexpr (LEAN) {
       const double v = pl_f64(p) ? r : (double)(float)r;   // the value as stored
       if (!is_bplane(p)) lout.set(p, c, v);                  // the limited value of a settled cell
-      else if (!interior) lout.set(p, c, 0.0);               // div_free_Balsara.f90:13 zeroes the halo B
       const int n = p < 48 ? p / 6 : (p < 50 ? 5 : 6);
       if (avg) pav[n] = v;
       else pa[n] += fabs(v) * pp_pmax(b);                    // b in increasing order for every n
@@ -257,13 +259,10 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 1) ? 4 : 1) void k_cell(
     for (int p = 0; p < NPL; p++) plane(p);
   }
   if constexpr (LEAN) {
-    bool need = false;
-    if (interior) {
-      const bool settled = !PP || pp_settled_sums(pa, pav, 1e-13);
-      if (settled) divfree_apply([&](int q, double v) { lout.set(q, c, v); }, Fn, g, i, j);
-      need = !settled;
+    if constexpr (PP) {
+      const bool need = interior && !pp_settled_sums(pa, pav, 1e-13);
+      pp_append(need, interior ? (j - 1) * g.Nx + (i - 1) : 0, pplist, lst);
     }
-    if constexpr (PP) pp_append(need, interior ? (j - 1) * g.Nx + (i - 1) : 0, pplist, lst);
   } else if constexpr (LIM) {
     // the limiter pass of k_limit<false, PP, true> on the stored stage value
     if (interior) {

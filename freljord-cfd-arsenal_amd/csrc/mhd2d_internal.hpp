@@ -296,6 +296,9 @@ int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const dou
 // the cells the mixed cell kernel's epilogue left to the positivity limiter
 // (the list of its bound test), limited from `in` into out
 int launch_pp_list(mhd2d_ctx* ctx, const double* in, double* out, const double* F);                 // mhd2d_limit.hip
+// div_free_Balsara into the B planes of out from faces F (halo B zeroed);
+// the other planes are left as they are
+int launch_divfree(mhd2d_ctx* ctx, double* out, const double* F);                                  // mhd2d_limit.hip
 int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, DevScal* sc,
                        int stage);                                                                           // mhd2d_limit.hip
 

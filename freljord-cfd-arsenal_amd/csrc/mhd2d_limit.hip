@@ -102,6 +102,29 @@ __global__ __launch_bounds__(BS) void k_pp_list(SRd<MIX> in, SWr<MIX> out, const
   }
 }
 
+// div_free_Balsara.f90:13-79 over the whole array, B planes only: the
+// in-cell B of every interior cell from its four new faces, the halo B
+// zeroed.  The limited B planes depend on the faces alone (positivity and
+// TVB changes of Bx, By are overwritten here), so in the mixed form this pass
+// runs after the CT update beside the cell kernel, whose lean epilogue writes
+// the other planes of the limited value
+template <bool MIX>
+__global__ __launch_bounds__(BS) void k_divfree(SWr<MIX> out, const double* __restrict__ F, Geo g) {
+  if (step_off(g)) return;
+  const long long t = xcd_block() * BS + threadIdx.x;
+  const int W = g.Nx + 2;
+  if (t >= (long long)W * (g.Ny + 2)) return;
+  const int i = (int)(t % W), j = (int)(t / W);
+  const long long c = cix(g, i, j);
+  if (i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny) {
+    divfree_apply([&](int p, double v) { out.set(p, c, v); }, F, g, i, j);
+  } else {
+#pragma unroll
+    for (int p = 0; p < NPL; p++)
+      if (is_bplane(p)) out.set(p, c, 0.0);
+  }
+}
+
 template <bool MIX>
 __global__ __launch_bounds__(BS) void k_kxrcf(SRd<MIX> U, int* __restrict__ trouble, Geo g,
                                               double Ck, double hk, DevScal* sc) {
@@ -399,6 +422,18 @@ int launch_pp_list(mhd2d_ctx* ctx, const double* in, double* out, const double* 
                      SWr<true>{out, g.plane}, F, (const int*)ctx->pplist, g, (const DevScal*)ctx->sc);
   LAUNCHCHK(ctx);
   return 0;
+}
+
+int launch_divfree(mhd2d_ctx* ctx, double* out, const double* F) {
+  const Geo& g = ctx->g;
+  Timed tm(ctx, KC_LIMIT);
+  const long long n = (long long)(g.Nx + 2) * (g.Ny + 2);
+  return with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_divfree<MIX>), dim3(nblk(n)), dim3(BS), 0, ctx->stream, SWr<MIX>{out, g.plane}, F, g);
+    LAUNCHCHK(ctx);
+    return 0;
+  });
 }
 
 int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, DevScal* sc,
