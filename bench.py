@@ -295,6 +295,8 @@ def main():
                     help="no timing: the dispatches a rocprofv3 --pmc pass reads (a streaming kernel of known "
                          "size, then two steps of each form)")
     args = ap.parse_args()
+    if args.profile_steps < 1 and not args.counters:
+        ap.error("--profile-steps must be at least 1: the roofline reads the event-timed kernel classes")
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no sharding, no collectives); use --gpus 1",
               file=sys.stderr)
