@@ -135,6 +135,7 @@ __global__ void k_dt_final(DevScal* sc, double hx, double hy, int advance) {
   if (!isfinite(dt)) sc->health |= MHD2D_FLAG_NONFINITE;
   if (!advance) { sc->dt = dt; return; }
   sc->active = 1;
+  sc->epoch += 1;
   if (dt < 1e-7) sc->health |= MHD2D_FLAG_DT_SMALL;   // the CFL step itself collapsed
   if (sc->t + dt >= sc->tend) { dt = sc->tend - sc->t; sc->t = sc->tend; }
   else sc->t = sc->t + dt;
@@ -638,6 +639,7 @@ int mhd2d_create(const mhd2d_config* cfg, mhd2d_ctx** out) {
       {(void**)&ctx->blist, (size_t)g.Nx * g.Ny * sizeof(int)},
       {(void**)&ctx->screen, (size_t)g.Nx * g.Ny * sizeof(int)},
       {(void**)&ctx->pplist, (size_t)g.Nx * g.Ny * sizeof(int)},
+      {(void**)&ctx->lflag, (size_t)g.Nx * g.Ny * sizeof(int)},
       {(void**)&ctx->trec, (size_t)32 * g.Nx * g.Ny * sizeof(double)},
       {(void**)&ctx->prob, (size_t)g.Nx * g.Ny * sizeof(float)},
       {(void**)&ctx->sc, sizeof(DevScal)},
@@ -683,7 +685,7 @@ int mhd2d_destroy(mhd2d_ctx* ctx) {
   ctx->ev_pool.~vector();
   void* ps[] = {ctx->stage, ctx->S, ctx->U0, ctx->UI, ctx->UII, ctx->T, ctx->du, ctx->F, ctx->F0, ctx->FI, ctx->FII,
                 ctx->Ft, ctx->dEz, ctx->FX, ctx->FY, ctx->EZX, ctx->EZY, ctx->EZV, ctx->trouble, ctx->tmask,
-                ctx->tlist, ctx->blist, ctx->screen, ctx->pplist, ctx->trec, ctx->prob, ctx->mlpw, ctx->sc,
+                ctx->tlist, ctx->blist, ctx->screen, ctx->pplist, ctx->lflag, ctx->trec, ctx->prob, ctx->mlpw, ctx->sc,
                 ctx->partial, ctx->icp};
   for (void* p : ps)
     if (p) (void)hipFree(p);
@@ -1404,7 +1406,9 @@ int step_body(mhd2d_ctx* ctx) {
       // mixed form: vertex EMFs beside the face kernels, then the CT update
       // and the divergence-free B planes of the limited value beside the
       // cell kernel (its lean epilogue writes the other planes and needs no
-      // faces)
+      // faces).  The stage value's planes other than Bx, By live in lim
+      // (its Bx, By planes in out) until the list kernels limit cells in
+      // place, so the detection reads lim and out and runs first.
       if ((r = wait(ctx->ev_fork, s1, s2))) return r;
       if ((r = on2(vertex))) return r;
       if ((r = launch_fluxes(ctx, ctx->S, false, sc))) return r;
@@ -1412,16 +1416,16 @@ int step_body(mhd2d_ctx* ctx) {
       if ((r = on2(ct)) || (r = on2([&] { return launch_divfree(ctx, lim, Fo); }))) return r;
       if (two) HIPCHK(ctx, hipEventRecord(ctx->ev_join, s2));       // new faces and limited B done
       if ((r = launch_cell(ctx, k, ctx->S, A, ctx->U0, out, sc, 0.0, lim, Fo, pp))) return r;
-      // detection and TVB on the second stream beside the positivity list;
-      // then the cells TVB changed are limited again (after the list, so
-      // that their TVB-limited value is the one that stays)
+      // detection and TVB on the second stream, then the positivity list and
+      // the cells TVB changed, limited again (after the list, so that their
+      // TVB-limited value is the one that stays)
       if ((r = wait(ctx->ev_cell, s1, s2))) return r;
-      if ((r = on2([&] { return launch_detection(ctx, out, sc, k); }))) return r;
+      if ((r = on2([&] { return launch_detection(ctx, lim, sc, k, out); }))) return r;
       if (two) HIPCHK(ctx, hipEventRecord(ctx->ev_det, s2));
       if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_join, 0));
-      if (pp && (r = launch_pp_list(ctx, out, lim, Fo))) return r;
       if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_det, 0));
-      return launch_limit_relist(ctx, out, lim, Fo, pp);
+      if (pp && (r = launch_pp_list(ctx, out, lim, Fo, k))) return r;
+      return launch_limit_relist(ctx, out, lim, Fo, pp, k, true);
     }
     // vertex EMFs and CT beside the memory-bound cell kernel
     if ((r = launch_fluxes(ctx, ctx->S, false, sc))) return r;
@@ -1442,7 +1446,7 @@ int step_body(mhd2d_ctx* ctx) {
     if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_join, 0));
     if ((r = launch_limit(ctx, out, lim, Fo, false, pp, true, sc, k))) return r;
     if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_det, 0));
-    return launch_limit_relist(ctx, out, lim, Fo, pp);
+    return launch_limit_relist(ctx, out, lim, Fo, pp, k);
   };
   // stage 1 (RK3.f90:74-97): set_bc; UI = S + dt L(S); S1 = limit(UI);
   // U0 keeps S with its fresh halos

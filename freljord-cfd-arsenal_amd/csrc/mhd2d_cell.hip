@@ -40,12 +40,14 @@ namespace {
 // 254 VGPRs, which halves the kernel's waves per SIMD and costs more than
 // the pass it saves (measured).  Instead the epilogue accumulates, plane by
 // plane as it stores them, the sums of the positivity bound (pp_settled) and
-// writes the planes other than Bx, By of the stage value as the limited
-// value; the cells the bound cannot settle are appended to pplist and limited
-// afterwards by a list kernel (k_pp_list).  The limited B planes come from
-// the faces alone (div_free_Balsara), so a separate pass (k_divfree) writes
-// them after the CT update, beside this kernel, and the kernel does not wait
-// for the CT.
+// writes the planes other than Bx, By of the stage value once, into the
+// limited value lout, which they are for every cell the bound settles; the
+// stage value's Bx, By planes go to out.  The cells the bound cannot settle
+// are appended to pplist and limited in place afterwards by a list kernel
+// (k_pp_list), which keeps their unlimited planes in out.  The limited B
+// planes come from the faces alone (div_free_Balsara), so a separate pass
+// (k_divfree) writes them after the CT update, beside this kernel, and the
+// kernel does not wait for the CT.
 template <int MODE, typename T, int CBS, bool SQ = false, bool MIX = (sizeof(T) == 4), bool LIM = false,
           bool PP = false>
 __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(SRd<MIX> S, const T* __restrict__ FX,
@@ -53,7 +55,8 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(
                                               SRd<MIX> U0, SWr<MIX> out, SWr<MIX> lout,
                                               const double* __restrict__ Fn, Geo g,
                                               const DevScal* __restrict__ sc, double dt_arg,
-                                              int* __restrict__ pplist = nullptr, DevScal* lst = nullptr) {
+                                              int* __restrict__ pplist = nullptr, DevScal* lst = nullptr,
+                                              const int* __restrict__ lflag = nullptr) {
   if (step_off(g)) return;
   // The mixed form keeps the first NR planes of the cell in registers and the
   // other 40 in LDS: 160 B per lane, four 256-lane blocks (16 waves) per CU
@@ -229,22 +232,55 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(
   double pa[LEAN ? 8 : 1], pav[LEAN ? 8 : 1];   // LEAN: positivity bound sums and the averages
 #pragma unroll
   for (int n = 0; n < (LEAN ? 8 : 1); n++) { pa[n] = 0.0; pav[n] = 0.0; }
+  // LEAN: the stage value's planes other than Bx, By are stored once, in the
+  // limited value lout (for every cell the list kernels do not limit, the
+  // two agree); out, its companion, holds its Bx, By planes, all planes of a
+  // halo cell (set_bc later refreshes lout's halos) and, written by the list
+  // kernels, the unlimited planes of the cells they limited in place (lflag).
+  // So the previous stage value A is read from out's predecessor only where
+  // aq; elsewhere its other planes are S's: the LDS copy (exact for the FP32
+  // planes) and a re-read of the FP64 cell averages.  An aq lane (rare)
+  // refills its LDS column with A's planes first, so that one code path
+  // serves both.
+  bool aq = true;
+  const double* sb = S.b;   // the buffer the epilogue re-reads the FP64 planes of S (or A) from
+  if constexpr (LEAN && MODE >= 2) {
+    const int tag = (int)(4 * sc->epoch + (MODE - 1));
+    aq = !interior || lflag[(long long)(j - 1) * g.Nx + (i - 1)] == tag;
+    if (aq) sb = A.b;
+    if (aq && interior) {
+      static_for<0, NPL>([&](auto P) {
+        constexpr int p = P;
+        if constexpr (!is_bplane(p) && !pl_f64(p)) {
+          const T v = T(A(p, c));
+          if constexpr (p < NR) rg[p < NR ? p : 0] = v;
+          else sm[(p - NR) * CBS + tid] = v;
+        }
+      });
+    }
+  }
   auto plane = [&](auto P) {
     const int p = P;
     const bool avg = p < 48 && (p % 6) == 0;
     const int b = p < 48 ? p % 6 : (p == 48 ? 6 : p == 49 ? 8 : p == 50 ? 7 : 9);
     const double du = avg ? acc0[avg ? p / 6 : 0] : (double)acc[p] * (1.0 / mmass(b));
+    // the LDS copy is exact for the planes stored in the quadrature type
+    auto s_exact = [&]() -> double {
+      return interior && (sizeof(T) == 8 || !pl_f64(p)) ? (double)(p < NR ? rg[p < NR ? p : 0] : sm[(p - NR) * CBS + tid])
+                                                         : SRd<MIX>{sb, pl}(p, c);
+    };
+    double a = 0.0;
+    if (MODE >= 2) a = (LEAN && !is_bplane(p)) ? s_exact() : A(p, c);
     double r;
     if (MODE == 0) r = du;
-    // the LDS copy is exact for the planes stored in the quadrature type
-    else if (MODE == 1) r = (interior && (sizeof(T) == 8 || !pl_f64(p)) ? (double)(p < NR ? rg[p < NR ? p : 0] : sm[(p - NR) * CBS + tid]) : S(p, c)) + dt * du;
-    else if (MODE == 2) r = (3.0 / 4.0) * U0(p, c) + (1.0 / 4.0) * (A(p, c) + dt * du);
-    else r = (1.0 / 3.0) * U0(p, c) + (2.0 / 3.0) * (A(p, c) + dt * du);
-    out.set(p, c, r);
+    else if (MODE == 1) r = s_exact() + dt * du;
+    else if (MODE == 2) r = (3.0 / 4.0) * U0(p, c) + (1.0 / 4.0) * (a + dt * du);
+    else r = (1.0 / 3.0) * U0(p, c) + (2.0 / 3.0) * (a + dt * du);
+    if (!LEAN || is_bplane(p) || !interior) out.set(p, c, r);
     if constexpr (LIM && !LEAN) m[p] = (MIX && !pl_f64(p)) ? (double)(float)r : r;
     if constexpr (LEAN) {
       const double v = pl_f64(p) ? r : (double)(float)r;   // the value as stored
-      if (!is_bplane(p)) lout.set(p, c, v);                  // the limited value of a settled cell
+      if (!is_bplane(p)) lout.set(p, c, v);                  // the stage value, also the limited value of a settled cell
       const int n = p < 48 ? p / 6 : (p < 50 ? 5 : 6);
       if (avg) pav[n] = v;
       else pa[n] += fabs(v) * pp_pmax(b);                    // b in increasing order for every n
@@ -423,13 +459,13 @@ void launch_cell_t(mhd2d_ctx* ctx, const double* S, const double* A, const doubl
                            dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)), dim3(CBS_F32), 0, ctx->stream,
                            SRd<true>{S, pl}, (const float*)ctx->FX, (const float*)ctx->FY, SRd<true>{A, pl},
                            SRd<true>{U0, pl}, SWr<true>{out, pl}, SWr<true>{lout, pl}, Fn, g, sc, dt, ctx->pplist,
-                           ctx->sc);
+                           ctx->sc, (const int*)ctx->lflag);
       else
         hipLaunchKernelGGL((k_cell<MODE, float, CBS_F32, false, true, true, PP>),
                            dim3((unsigned)((n + CBS_F32 - 1) / CBS_F32)), dim3(CBS_F32), 0, ctx->stream,
                            SRd<true>{S, pl}, (const float*)ctx->FX, (const float*)ctx->FY, SRd<true>{A, pl},
                            SRd<true>{U0, pl}, SWr<true>{out, pl}, SWr<true>{lout, pl}, Fn, g, sc, dt, ctx->pplist,
-                           ctx->sc);
+                           ctx->sc, (const int*)ctx->lflag);
       return;
     }
     hipLaunchKernelGGL((k_cell<MODE, double, CBS_F64, false, false, true, PP>),

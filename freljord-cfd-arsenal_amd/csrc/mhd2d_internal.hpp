@@ -38,6 +38,8 @@ struct DevScal {
   unsigned int npp;     // length of the list of cells the positivity limiter must look at (mixed form)
   int active;
   int health;           // MHD2D_FLAG_* bits, sticky until mhd2d_set_time
+  long long epoch;      // passes that advanced, since the context was created (never reset): the
+                        // mixed form's limiter lists mark cells with 4 epoch + stage (lflag)
 };
 
 __device__ __forceinline__ long long dkey(double x) {   // order-preserving key of a double
@@ -167,6 +169,7 @@ struct mhd2d_ctx {
   int* blist;          // interior-cell indices the indicator hands to KXRCF (sc->nband)
   int* screen;         // indicator: 1 where p >= mlp_lo (before the one-cell dilation)
   int* pplist;         // mixed form: interior cells whose positivity bound does not settle them (sc->npp)
+  int* lflag;          // mixed form: 4 epoch + stage where a list kernel limited the cell in place
   double* trec;        // modified edge values of changed cells: 32 planes over the interior cells
   void* mlpw;          // indicator MLP weights in MFMA operand order (mhd2d_mlp.hip)
   float* prob;         // indicator MLP probability per interior cell
@@ -278,11 +281,11 @@ int launch_axpy(mhd2d_ctx* ctx, double* out, const double* X, double a, const do
 int launch_axpy_state(mhd2d_ctx* ctx, double* out, const double* X, double a, const double* Y, double b,
                       const double* Z, double dt);                                            // mhd2d_cell.hip
 int launch_kxrcf(mhd2d_ctx* ctx, const double* U, DevScal* sc);                               // mhd2d_limit.hip
-int launch_tvb_list(mhd2d_ctx* ctx, const double* in);                                       // mhd2d_limit.hip
-int launch_detect_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc = nullptr);              // mhd2d_limit.hip
+int launch_tvb_list(mhd2d_ctx* ctx, const double* in, const double* inB = nullptr);                                       // mhd2d_limit.hip
+int launch_detect_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc = nullptr, const double* inB = nullptr);              // mhd2d_limit.hip
 int launch_tvb_all(mhd2d_ctx* ctx, const double* in, int* kx);                               // mhd2d_limit.hip
-int launch_mlp_screen(mhd2d_ctx* ctx, const double* in);   // indicator: prob[], screen, candidate list   // mhd2d_mlp.hip
-int launch_candidates_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc);  // KXRCF + TVB on the candidates  // mhd2d_mlp.hip
+int launch_mlp_screen(mhd2d_ctx* ctx, const double* in, const double* inB = nullptr);   // indicator: prob[], screen, candidate list   // mhd2d_mlp.hip
+int launch_candidates_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc, const double* inB = nullptr);  // KXRCF + TVB on the candidates  // mhd2d_mlp.hip
 // stage: 1..3 inside mhd2d_step (the learned indicator screens once, at
 // stage 1, and reuses the screen for stages 2 and 3); 0 for a standalone call
 int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool tvb, bool pp, bool div,
@@ -291,11 +294,12 @@ int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F,
 // epilogue, or k_limit<false, ..>): detection and TVB on the unlimited stage
 // value `in` (KXRCF-gated and learned indicators), then the cells TVB changed
 // limited again from `in` into out.  launch_limit_fixup = both.
-int launch_detection(mhd2d_ctx* ctx, const double* in, DevScal* sc, int stage);                    // mhd2d_limit.hip
-int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp);  // mhd2d_limit.hip
+int launch_detection(mhd2d_ctx* ctx, const double* in, DevScal* sc, int stage, const double* inB = nullptr);                    // mhd2d_limit.hip
+int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, int stage,
+                        bool lean = false);  // mhd2d_limit.hip
 // the cells the mixed cell kernel's epilogue left to the positivity limiter
 // (the list of its bound test), limited from `in` into out
-int launch_pp_list(mhd2d_ctx* ctx, const double* in, double* out, const double* F);                 // mhd2d_limit.hip
+int launch_pp_list(mhd2d_ctx* ctx, double* q, double* out, const double* F, int stage);                 // mhd2d_limit.hip
 // div_free_Balsara into the B planes of out from faces F (halo B zeroed);
 // the other planes are left as they are
 int launch_divfree(mhd2d_ctx* ctx, double* out, const double* F);                                  // mhd2d_limit.hip

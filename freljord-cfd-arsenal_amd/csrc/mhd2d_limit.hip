@@ -13,8 +13,8 @@ namespace {
 // limiter of one cell (TVB_Limiter_P2.f90:16-121 + pp + div_free_Balsara):
 // interior cells are limited, halo cells copied (or, with DIV, their B
 // planes zeroed as div_free_Balsara.f90:13 does)
-template <bool APPLY, bool PP, bool DIV, bool MIX>
-__device__ __forceinline__ void limit_cell(const SRd<MIX>& in, const SWr<MIX>& out, const double* __restrict__ F,
+template <bool APPLY, bool PP, bool DIV, bool MIX, class IV>
+__device__ __forceinline__ void limit_cell(const IV& in, const SWr<MIX>& out, const double* __restrict__ F,
                                            const int* __restrict__ tmask, const double* __restrict__ trec,
                                            const Geo& g, int i, int j) {
   const long long c = cix(g, i, j);
@@ -68,37 +68,63 @@ __global__ __launch_bounds__(BS) void k_limit(SRd<MIX> in, SWr<MIX> out,
   limit_cell<APPLY, PP, DIV, MIX>(in, out, F, tmask, trec, g, (int)(t % W), (int)(t / W));
 }
 
+// The mixed form's lean stage stores the stage value's planes other than
+// Bx, By once, in the limited value (out); its companion (q) holds the Bx, By
+// planes.  Before a list kernel limits interior cell tc in place, the cell's
+// other unlimited planes are copied to the companion and the cell is marked
+// with this stage's tag 4 epoch + stage (the next stage's cell kernel then
+// reads its RK input there); the companion then holds the whole unlimited
+// value of the cell.
+template <bool MIX>
+__device__ __forceinline__ void lean_take(const SWr<MIX>& q, const SWr<MIX>& out, int* __restrict__ lflag, int tag,
+                                          const Geo& g, int tc) {
+  if (lflag[tc] == tag) return;   // a list kernel of this stage took it already
+  const long long c = cix(g, 1 + tc % g.Nx, 1 + tc / g.Nx);
+#pragma unroll
+  for (int p = 0; p < NPL; p++)
+    if (!is_bplane(p)) q.set(p, c, out(p, c));
+  lflag[tc] = tag;
+}
+
 // the cells of the flagged list whose TVB changed something: the cell
 // kernel's fused epilogue limited every cell without TVB (pp and the
 // divergence-free reconstruction), so these are limited again from the
-// unlimited stage value, with TVB (a fixed grid strides the list)
-template <bool PP, bool MIX>
+// unlimited stage value, with TVB (a fixed grid strides the list).  LEAN:
+// the mixed form's lean stage (in is the companion, lean_take)
+template <bool PP, bool MIX, bool LEAN>
 __global__ __launch_bounds__(BS) void k_limit_list(SRd<MIX> in, SWr<MIX> out, const double* __restrict__ F,
                                                    const int* __restrict__ list, const int* __restrict__ tmask,
                                                    const double* __restrict__ trec, Geo g,
-                                                   const DevScal* __restrict__ lst) {
+                                                   const DevScal* __restrict__ lst, int* __restrict__ lflag,
+                                                   int stage) {
   if (step_off(g)) return;
   const unsigned int n = lst->nlist;
+  const int tag = (int)(4 * lst->epoch + stage);
   for (unsigned int k = blockIdx.x * BS + threadIdx.x; k < n; k += gridDim.x * BS) {
     const int tc = list[k];
     if (!tmask[tc]) continue;
+    if (LEAN) lean_take(SWr<MIX>{const_cast<double*>(in.b), in.pl}, out, lflag, tag, g, tc);
     limit_cell<true, PP, true, MIX>(in, out, F, tmask, trec, g, 1 + tc % g.Nx, 1 + tc / g.Nx);
   }
 }
 
 // the cells the mixed cell kernel's epilogue could not settle with the
 // positivity bound: the full limiter pass without TVB (positivity, then the
-// divergence-free reconstruction) from the unlimited stage value; the cells
-// TVB changes are limited again afterwards (k_limit_list)
+// divergence-free reconstruction) from the unlimited stage value (in, the
+// companion, after lean_take); the cells TVB changes are limited again
+// afterwards (k_limit_list)
 template <bool MIX>
-__global__ __launch_bounds__(BS) void k_pp_list(SRd<MIX> in, SWr<MIX> out, const double* __restrict__ F,
+__global__ __launch_bounds__(BS) void k_pp_list(SWr<MIX> q, SWr<MIX> out, const double* __restrict__ F,
                                                 const int* __restrict__ list, Geo g,
-                                                const DevScal* __restrict__ lst) {
+                                                const DevScal* __restrict__ lst, int* __restrict__ lflag,
+                                                int stage) {
   if (step_off(g)) return;
   const unsigned int n = lst->npp;
+  const int tag = (int)(4 * lst->epoch + stage);
   for (unsigned int k = blockIdx.x * BS + threadIdx.x; k < n; k += gridDim.x * BS) {
     const int tc = list[k];
-    limit_cell<false, true, true, MIX>(in, out, F, nullptr, nullptr, g, 1 + tc % g.Nx, 1 + tc / g.Nx);
+    lean_take(q, out, lflag, tag, g, tc);
+    limit_cell<false, true, true, MIX>(SRd<MIX>(q), out, F, nullptr, nullptr, g, 1 + tc % g.Nx, 1 + tc / g.Nx);
   }
 }
 
@@ -145,7 +171,7 @@ __global__ void k_list_reset(DevScal* lst, const int* gate) {
 // KXRCF over the interior cells for the limiter pass: trouble[] and the
 // compacted list of flagged cells (one atomic per wave), tmask[] cleared
 template <bool MIX>
-__global__ __launch_bounds__(BS) void k_detect(SRd<MIX> U, int* __restrict__ trouble,
+__global__ __launch_bounds__(BS) void k_detect(SRdB<MIX> U, int* __restrict__ trouble,
                                                int* __restrict__ tmask, int* __restrict__ list, Geo g, double Ck,
                                                double hk, DevScal* sc, DevScal* lst) {
   if (step_off(g)) return;
@@ -243,7 +269,7 @@ __global__ __launch_bounds__(BS) void k_tvb(SRd<MIX> in, int* __restrict__ troub
 
 // TVB on the listed (flagged) cells only; a fixed grid strides the list
 template <bool MIX>
-__global__ __launch_bounds__(BS) void k_tvb_list(SRd<MIX> in, const int* __restrict__ list,
+__global__ __launch_bounds__(BS) void k_tvb_list(SRdB<MIX> in, const int* __restrict__ list,
                                                  int* __restrict__ tmask, double* __restrict__ trec, Geo g,
                                                  double M, const DevScal* __restrict__ lst) {
   if (step_off(g)) return;
@@ -282,12 +308,12 @@ int launch_kxrcf(mhd2d_ctx* ctx, const double* U, DevScal* sc) {
 }
 
 // TVB over the compacted list of flagged cells (a fixed grid strides it)
-int launch_tvb_list(mhd2d_ctx* ctx, const double* in) {
+int launch_tvb_list(mhd2d_ctx* ctx, const double* in, const double* inB) {
   const Geo& g = ctx->g;
   const unsigned nb = std::min<unsigned>(nblk((long long)g.Nx * g.Ny), 1024u);
   return with_mix(ctx, [&](auto Mc) -> int {
     constexpr bool MIX = decltype(Mc)::value;
-    hipLaunchKernelGGL((k_tvb_list<MIX>), dim3(nb), dim3(BS), 0, ctx->stream, SRd<MIX>{in, g.plane},
+    hipLaunchKernelGGL((k_tvb_list<MIX>), dim3(nb), dim3(BS), 0, ctx->stream, SRdB<MIX>{in, g.plane, inB},
                        (const int*)ctx->tlist, ctx->tmask, ctx->trec, g, ctx->cfg.M, (const DevScal*)ctx->sc);
     LAUNCHCHK(ctx);
     return 0;
@@ -296,19 +322,19 @@ int launch_tvb_list(mhd2d_ctx* ctx, const double* in) {
 
 // KXRCF with a compacted list, then TVB over the list: trouble[] = KXRCF
 // flags, tmask[] = TVB change bits of the flagged cells (0 elsewhere)
-int launch_detect_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc) {
+int launch_detect_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc, const double* inB) {
   const Geo& g = ctx->g;
   hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, ctx->g.gate);
   int rc = with_mix(ctx, [&](auto Mc) -> int {
     constexpr bool MIX = decltype(Mc)::value;
     hipLaunchKernelGGL((k_detect<MIX>), dim3(nblk((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream,
-                       SRd<MIX>{in, g.plane}, ctx->trouble, ctx->tmask, ctx->tlist, g, ctx->cfg.Ck, kx_hk(g), sc,
-                       ctx->sc);
+                       SRdB<MIX>{in, g.plane, inB}, ctx->trouble, ctx->tmask, ctx->tlist, g, ctx->cfg.Ck, kx_hk(g),
+                       sc, ctx->sc);
     LAUNCHCHK(ctx);
     return 0;
   });
   if (rc) return rc;
-  return launch_tvb_list(ctx, in);
+  return launch_tvb_list(ctx, in, inB);
 }
 
 // KXRCF flags of every cell into kx[], TVB change bits of every cell into
@@ -378,21 +404,23 @@ int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F,
   });
 }
 
-int launch_detection(mhd2d_ctx* ctx, const double* in, DevScal* sc, int stage) {
+int launch_detection(mhd2d_ctx* ctx, const double* in, DevScal* sc, int stage, const double* inB) {
   const int ind = ctx->cfg.indicator;
   if (ind != MHD2D_IND_MLP && ind != MHD2D_IND_KXRCF) return 0;
   Timed tm(ctx, KC_LIMIT);
   int rc;
   if (ind == MHD2D_IND_MLP) {
-    if (stage <= 1 && (rc = launch_mlp_screen(ctx, in))) return rc;
-    return launch_candidates_tvb(ctx, in, sc);
+    if (stage <= 1 && (rc = launch_mlp_screen(ctx, in, inB))) return rc;
+    return launch_candidates_tvb(ctx, in, sc, inB);
   }
-  return launch_detect_tvb(ctx, in, sc);
+  return launch_detect_tvb(ctx, in, sc, inB);
 }
 
-int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp) {
+int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, int stage,
+                        bool lean) {
   const int ind = ctx->cfg.indicator;
   if (ind != MHD2D_IND_MLP && ind != MHD2D_IND_KXRCF) return 0;
+  if (lean && ctx->cfg.precision != MHD2D_MIXED) return fail(ctx, -1, "launch_limit_relist: lean stage in FP64");
   const Geo& g = ctx->g;
   Timed tm(ctx, KC_LIMIT);
   const unsigned nb = std::min<unsigned>(nblk((long long)g.Nx * g.Ny), 1024u);
@@ -400,26 +428,31 @@ int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const dou
     constexpr bool MIX = decltype(Mc)::value;
     const SRd<MIX> vi{in, g.plane};
     const SWr<MIX> vo{out, g.plane};
-    if (pp)
-      hipLaunchKernelGGL((k_limit_list<true, MIX>), dim3(nb), dim3(BS), 0, ctx->stream, vi, vo, F,
+    auto go = [&](auto Pc, auto Lc) {
+      constexpr bool P = decltype(Pc)::value, L = decltype(Lc)::value && MIX;
+      hipLaunchKernelGGL((k_limit_list<P, MIX, L>), dim3(nb), dim3(BS), 0, ctx->stream, vi, vo, F,
                          (const int*)ctx->tlist, (const int*)ctx->tmask, (const double*)ctx->trec, g,
-                         (const DevScal*)ctx->sc);
-    else
-      hipLaunchKernelGGL((k_limit_list<false, MIX>), dim3(nb), dim3(BS), 0, ctx->stream, vi, vo, F,
-                         (const int*)ctx->tlist, (const int*)ctx->tmask, (const double*)ctx->trec, g,
-                         (const DevScal*)ctx->sc);
+                         (const DevScal*)ctx->sc, ctx->lflag, stage);
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    if (pp && lean) go(T_{}, T_{});
+    else if (pp) go(T_{}, F_{});
+    else if (lean) go(F_{}, T_{});
+    else go(F_{}, F_{});
     LAUNCHCHK(ctx);
     return 0;
   });
 }
 
-int launch_pp_list(mhd2d_ctx* ctx, const double* in, double* out, const double* F) {
+int launch_pp_list(mhd2d_ctx* ctx, double* q, double* out, const double* F, int stage) {
   if (ctx->cfg.precision != MHD2D_MIXED) return 0;
   const Geo& g = ctx->g;
   Timed tm(ctx, KC_LIMIT);
   const unsigned nb = std::min<unsigned>(nblk((long long)g.Nx * g.Ny), 1024u);
-  hipLaunchKernelGGL((k_pp_list<true>), dim3(nb), dim3(BS), 0, ctx->stream, SRd<true>{in, g.plane},
-                     SWr<true>{out, g.plane}, F, (const int*)ctx->pplist, g, (const DevScal*)ctx->sc);
+  hipLaunchKernelGGL((k_pp_list<true>), dim3(nb), dim3(BS), 0, ctx->stream, SWr<true>{q, g.plane},
+                     SWr<true>{out, g.plane}, F, (const int*)ctx->pplist, g, (const DevScal*)ctx->sc, ctx->lflag,
+                     stage);
   LAUNCHCHK(ctx);
   return 0;
 }
@@ -442,7 +475,7 @@ int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const doub
   if (ind == MHD2D_IND_NONE) return 0;
   if (ind == MHD2D_IND_ALL) return fail(ctx, -1, "launch_limit_fixup: TVB on every cell has no fused limiter");
   int rc = launch_detection(ctx, in, sc, stage);
-  return rc ? rc : launch_limit_relist(ctx, in, out, F, pp);
+  return rc ? rc : launch_limit_relist(ctx, in, out, F, pp, stage);
 }
 
 }  // namespace mhd2d_impl

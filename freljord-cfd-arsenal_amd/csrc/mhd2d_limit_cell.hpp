@@ -225,6 +225,4 @@ __device__ __forceinline__ void pp_append(bool flag, int t, int* __restrict__ li
   }
 }
 
-__device__ __forceinline__ bool is_bplane(int p) { return (p >= 30 && p < 42) || p >= 48; }
-
 }  // namespace mhd2d_impl

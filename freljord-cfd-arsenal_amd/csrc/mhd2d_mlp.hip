@@ -166,7 +166,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 template <bool MIX>
-__global__ __launch_bounds__(256) void k_mlp_screen(SRd<MIX> U, int* __restrict__ screen,
+__global__ __launch_bounds__(256) void k_mlp_screen(SRdB<MIX> U, int* __restrict__ screen,
                                                     float* __restrict__ prob, const MlpDev* __restrict__ W,
                                                     Geo g, float l10Mh2, float l10h, float lo, float hi) {
   if (step_off(g)) return;
@@ -294,7 +294,7 @@ __global__ void k_reset_nlist(DevScal* lst, const int* gate) {
 // list, and so do the cells the network itself decides (screen mark 2)
 // without the KXRCF evaluation
 template <bool MIX>
-__global__ __launch_bounds__(BS) void k_kxrcf_cands(SRd<MIX> U, int* __restrict__ trouble,
+__global__ __launch_bounds__(BS) void k_kxrcf_cands(SRdB<MIX> U, int* __restrict__ trouble,
                                                     int* __restrict__ tmask, int* __restrict__ list,
                                                     const int* __restrict__ blist, const int* __restrict__ screen,
                                                     Geo g, double Ck, double hk, DevScal* sc, DevScal* lst) {
@@ -529,14 +529,14 @@ int mlp_upload(mhd2d_ctx* ctx, const float* params) {
 
 static inline unsigned nb_blocks(long long n) { return (unsigned)((n + BS - 1) / BS); }
 
-static int launch_screen_only(mhd2d_ctx* ctx, const double* in) {
+static int launch_screen_only(mhd2d_ctx* ctx, const double* in, const double* inB = nullptr) {
   const Geo& g = ctx->g;
   if (!ctx->mlpw) return fail(ctx, -6, "indicator MLP: no weights (mhd2d_mlp_set_weights)");
   const long long ncell = (long long)g.Nx * g.Ny;
   return with_mix(ctx, [&](auto Mc) -> int {
     constexpr bool MIX = decltype(Mc)::value;
     hipLaunchKernelGGL((k_mlp_screen<MIX>), dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, ctx->stream,
-                       SRd<MIX>{in, g.plane}, ctx->screen, ctx->prob, (const MlpDev*)ctx->mlpw, g,
+                       SRdB<MIX>{in, g.plane, inB}, ctx->screen, ctx->prob, (const MlpDev*)ctx->mlpw, g,
                        (float)log10(ctx->cfg.M * g.hx * g.hy), (float)(0.5 * log10(sqrt(g.hx * g.hy))),
                        (float)ctx->cfg.mlp_lo, (float)ctx->cfg.mlp_hi);
     LAUNCHCHK(ctx);
@@ -544,9 +544,9 @@ static int launch_screen_only(mhd2d_ctx* ctx, const double* in) {
   });
 }
 
-int launch_mlp_screen(mhd2d_ctx* ctx, const double* in) {
+int launch_mlp_screen(mhd2d_ctx* ctx, const double* in, const double* inB) {
   const Geo& g = ctx->g;
-  int rc = launch_screen_only(ctx, in);
+  int rc = launch_screen_only(ctx, in, inB);
   if (rc) return rc;
   hipLaunchKernelGGL(k_reset_nband, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, ctx->g.gate);
   hipLaunchKernelGGL(k_dilate, dim3(nb_blocks((long long)g.Nx * g.Ny)), dim3(BS), 0, ctx->stream,
@@ -555,21 +555,21 @@ int launch_mlp_screen(mhd2d_ctx* ctx, const double* in) {
   return 0;
 }
 
-int launch_candidates_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc) {
+int launch_candidates_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc, const double* inB) {
   const Geo& g = ctx->g;
   const long long ncell = (long long)g.Nx * g.Ny;
   const unsigned nb = std::min<unsigned>(nb_blocks(ncell), 1024u);
   hipLaunchKernelGGL(k_reset_nlist, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, ctx->g.gate);
   int rc = with_mix(ctx, [&](auto Mc) -> int {
     constexpr bool MIX = decltype(Mc)::value;
-    hipLaunchKernelGGL((k_kxrcf_cands<MIX>), dim3(nb), dim3(BS), 0, ctx->stream, SRd<MIX>{in, g.plane}, ctx->trouble,
+    hipLaunchKernelGGL((k_kxrcf_cands<MIX>), dim3(nb), dim3(BS), 0, ctx->stream, SRdB<MIX>{in, g.plane, inB}, ctx->trouble,
                        ctx->tmask, ctx->tlist, (const int*)ctx->blist, (const int*)ctx->screen, g, ctx->cfg.Ck,
                        kx_hk(g), sc, ctx->sc);
     LAUNCHCHK(ctx);
     return 0;
   });
   if (rc) return rc;
-  return launch_tvb_list(ctx, in);
+  return launch_tvb_list(ctx, in, inB);
 }
 
 }  // namespace mhd2d_impl

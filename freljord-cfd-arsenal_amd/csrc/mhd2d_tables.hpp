@@ -163,6 +163,8 @@ __host__ __device__ constexpr int slot_d(int p) {
   return p < 30 ? p / 6 : p < 42 ? 5 + (p - 30) : p == 42 ? 17 : 18 + (p - 48);
 }
 __host__ __device__ constexpr int slot_f(int p) { return p < 30 ? (p / 6) * 5 + (p % 6 - 1) : 25 + (p - 43); }
+// the planes of Bx and By (all their modes)
+__host__ __device__ constexpr bool is_bplane(int p) { return (p >= 30 && p < 42) || p >= 48; }
 
 struct Geo {
   int Nx, Ny, P;
@@ -219,6 +221,18 @@ struct SRd {
     if (!MIX) return b[(long long)p * pl + c];
     if (pl_f64(p)) return b[(long long)slot_d(p) * pl + c];
     return (double)reinterpret_cast<const float*>(b + NPL_D * pl)[(long long)slot_f(p) * pl + c];
+  }
+};
+// the same over two buffers of that layout: the Bx, By planes from bq (when
+// set), the others from b.  The mixed form's unlimited stage value is read
+// so: its planes other than Bx, By share the limited value's storage.
+template <bool MIX>
+struct SRdB {
+  const double* b;
+  long long pl;
+  const double* bq;
+  __device__ __forceinline__ double operator()(int p, long long c) const {
+    return SRd<MIX>{(bq && is_bplane(p)) ? bq : b, pl}(p, c);
   }
 };
 template <bool MIX>

@@ -429,7 +429,11 @@ def test_mixed_shock_rotor(pkg):
     close to the plain form (measured 6.1e-6 in density after 20 steps at
     128^2; the bar is about three times that).  FP32 rounding can move a
     limiter decision at a discontinuity, so the bar is looser than the smooth
-    benchmark flow's 1e-5 (test_mixed_vs_fp64)."""
+    benchmark flow's 1e-5 (test_mixed_vs_fp64).  The mixed form's list
+    kernels limit the cells TVB changes in place and keep their unlimited
+    value apart for the next stage's RK combination; combining the limited
+    value instead would move every such cell by a share of the limiter's
+    change, far past the bar."""
     N = 128
     out = {}
     for prec in (pkg.FP64, pkg.MIXED):
