@@ -58,13 +58,19 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(
                                               int* __restrict__ pplist = nullptr, DevScal* lst = nullptr,
                                               const int* __restrict__ lflag = nullptr) {
   if (step_off(g)) return;
-  // The mixed form keeps the first NR planes of the cell in registers and the
-  // other 40 in LDS: 160 B per lane, four 256-lane blocks (16 waves) per CU
-  // where 52 planes (208 B per lane) allowed three; the stage 2 and 3 kernels
-  // stay at 126 VGPRs, so a SIMD runs four waves instead of three (1.20 ->
-  // 1.13 ms per launch at 2048^2, measured).  More planes in registers would
-  // need a fifth wave's 102-VGPR limit, which spills.
+  // The mixed form keeps NR planes of the cell in registers and the other 40
+  // in LDS: 160 B per lane, four 256-lane blocks (16 waves) per CU where 52
+  // planes (208 B per lane) allowed three, so a SIMD runs four waves instead
+  // of three (1.20 -> 1.13 ms per launch at 2048^2, measured).  More planes in
+  // registers would need a fifth wave's 102-VGPR limit, which spills.  The
+  // register planes are Bx and By modes 1-6 (planes 30-41): the epilogue
+  // re-reads the exact FP64 B planes of S or takes A's, so the registers are
+  // free once the quadrature is done, while the FP32 planes the epilogue takes
+  // from the copy (the stage value's other planes, shared with S) stay in LDS.
   constexpr int NR = MIX ? 12 : 0;
+  constexpr int R0 = 30;   // first register plane
+  auto in_rg = [](int p) { return NR > 0 && p >= R0 && p < R0 + NR; };
+  auto lds_slot = [](int p) { return NR > 0 && p >= R0 + NR ? p - NR : p; };
   __shared__ T sm[(NPL - NR) * CBS];
   T rg[NR > 0 ? NR : 1];
   const int tid = threadIdx.x;
@@ -85,8 +91,8 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(
   if (interior) {
 #pragma unroll
     for (int p = 0; p < NPL; p++) {
-      if (p < NR) rg[p < NR ? p : 0] = T(S(p, c));
-      else sm[(p - NR) * CBS + tid] = T(S(p, c));
+      if (in_rg(p)) rg[in_rg(p) ? p - R0 : 0] = T(S(p, c));
+      else sm[lds_slot(p) * CBS + tid] = T(S(p, c));
     }
     const T ihx1 = T(1.0 / g.hx1), ihy1 = T(1.0 / g.hy1);
     // one quadrature row; the mixed form runs rows and columns as
@@ -99,7 +105,7 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(
       int off = tid;
       asm volatile("" : "+v"(off));   // re-read the modes from LDS for every row
       const T* ms = sm + off;
-      auto mode = [&](int p) -> T { return p < NR ? rg[p < NR ? p : 0] : ms[(p - NR) * CBS]; };
+      auto mode = [&](int p) -> T { return in_rg(p) ? rg[in_rg(p) ? p - R0 : 0] : ms[lds_slot(p) * CBS]; };
       // U(x, y_j) = c0 + c1 x + c2 P2(x) + c3 P3(x) per variable
       T c0[8], c1[8], c2[8], c3b = T(0);
 #pragma unroll
@@ -242,19 +248,14 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(
   // planes) and a re-read of the FP64 cell averages.  An aq lane (rare)
   // refills its LDS column with A's planes first, so that one code path
   // serves both.
-  bool aq = true;
-  const double* sb = S.b;   // the buffer the epilogue re-reads the FP64 planes of S (or A) from
   if constexpr (LEAN && MODE >= 2) {
     const int tag = (int)(4 * sc->epoch + (MODE - 1));
-    aq = !interior || lflag[(long long)(j - 1) * g.Nx + (i - 1)] == tag;
-    if (aq) sb = A.b;
-    if (aq && interior) {
+    if (interior && lflag[(long long)(j - 1) * g.Nx + (i - 1)] == tag) {
       static_for<0, NPL>([&](auto P) {
         constexpr int p = P;
         if constexpr (!is_bplane(p) && !pl_f64(p)) {
           const T v = T(A(p, c));
-          if constexpr (p < NR) rg[p < NR ? p : 0] = v;
-          else sm[(p - NR) * CBS + tid] = v;
+          sm[lds_slot(p) * CBS + tid] = v;   // no register plane holds one of these
         }
       });
     }
@@ -266,8 +267,8 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(
     const double du = avg ? acc0[avg ? p / 6 : 0] : (double)acc[p] * (1.0 / mmass(b));
     // the LDS copy is exact for the planes stored in the quadrature type
     auto s_exact = [&]() -> double {
-      return interior && (sizeof(T) == 8 || !pl_f64(p)) ? (double)(p < NR ? rg[p < NR ? p : 0] : sm[(p - NR) * CBS + tid])
-                                                         : SRd<MIX>{sb, pl}(p, c);
+      return interior && (sizeof(T) == 8 || !pl_f64(p)) ? (double)(in_rg(p) ? rg[in_rg(p) ? p - R0 : 0] : sm[lds_slot(p) * CBS + tid])
+                                                         : ((LEAN && MODE >= 2 && !interior) ? A(p, c) : S(p, c));
     };
     double a = 0.0;
     if (MODE >= 2) a = (LEAN && !is_bplane(p)) ? s_exact() : A(p, c);
