@@ -9,7 +9,8 @@
   continues from the right buffers
 - the mixed form's double-double CT keeps div B from accumulating
 - weights set after the capture take effect in the replays
-- the stage's second stream gives the same bits as the one-stream step
+- the stage's second stream gives the same bits as the one-stream step,
+  also on a flow whose limiter lists are busy (the rotor)
 - closing a context gives its device memory back
 """
 import numpy as np
@@ -127,6 +128,41 @@ def test_two_streams_equal_one_stream(pkg, form):
     c.step_graph(7)
     a.profile(False)
     assert same(a, b) and same(a, c)
+
+
+@pytest.mark.parametrize("form", ["mixed_kxrcf", "mixed_nn", "fp64"])
+def test_two_streams_equal_one_stream_rotor(pkg, form):
+    """the same on the reference's rotor (init4.txt, positivity, TVB M = 10
+    gated by KXRCF or the screen), where TVB changes cells at every stage:
+    the mixed form's list kernels limit those cells in place after the
+    detection has read the stage value, and the next stage combines their
+    unlimited value from the companion buffer; the two-stream order, the
+    one-stream order and the graph replays must give the same bits"""
+    import importlib
+    prec, ind = {"fp64": (pkg.FP64, pkg.IND_KXRCF), "mixed_nn": (pkg.MIXED, pkg.IND_MLP),
+                 "mixed_kxrcf": (pkg.MIXED, pkg.IND_KXRCF)}[form]
+    d = pkg.ic_defaults(pkg.IC_ROTOR)
+    W = pkg.load_indicator_weights()
+    z = np.load(pkg.WEIGHTS_PATH, allow_pickle=False)
+
+    def mk():
+        s = pkg.Solver(64, 64, flux_type=pkg.HLLD, bc=d["bc"], domain=d["domain"], M=10.0, CFL=0.15, pp=1,
+                       indicator=ind, precision=prec, Ck=1.0, mlp_band=(float(z["mlp_lo"]), float(z["mlp_hi"])))
+        s.mlp_set_weights(W)
+        s.init_data(pkg.IC_ROTOR)
+        s.rk3_prologue()
+        return s
+    a, b, c = mk(), mk(), mk()
+    a.profile(True)
+    for _ in range(12):
+        a.step()
+        b.step()
+    c.step_graph(12)
+    a.profile(False)
+    assert same(a, b) and same(a, c)
+    st = importlib.import_module("freljord_cfd_arsenal_amd.indicator").limiter_input_stats(a)
+    print(f"[rotor] {form}: after 12 steps KXRCF flags {st['kxrcf']}, TVB changes {st['label']}")
+    assert st["label"] > 0          # the flow keeps the limiter lists busy
 
 
 def test_contexts_free_their_memory(pkg):
