@@ -99,6 +99,28 @@ def test_vortex_convergence(pkg):
     assert all(r["divB_fp64"] < 1e-12 and r["divB_mixed"] < 1e-12 for r in rows)
 
 
+def test_vortex_precision_config(pkg):
+    """the precision configuration at its own domain and end time: the
+    smooth vortex on [-5, 5]^2 to t = 1.0 at 256^2, 512^2 and 1024^2 (the
+    2048^2 row, 2.5 more minutes, is in the committed precision profile).
+    FP64 converges at third order in density; the mixed form stays within
+    1e-5 of FP64 in every field (measured 1.5e-6 in density and 5.5e-6 in
+    Bx, By at 1024^2) and keeps div B at roundoff.  On this domain the
+    vortex's tail is cut by the periodic boundary, which fixes the momentum
+    and energy errors of both forms at 1.7-2.4e-4, and the mixed form's own
+    density error sits on its FP32 floor (7e-6 to 1.5e-5), so neither order
+    is asserted there."""
+    ps = importlib.import_module("freljord_cfd_arsenal_amd.precision_study")
+    rows = ps.study((256, 512, 1024), 1.0, domain=(-5.0, 5.0, -5.0, 5.0))
+    for r in rows:
+        print(f"[configs] vortex [-5,5]^2 t=1 N={r['N']}: L2 rho FP64 {r['L2_fp64'][0]:.3e} mixed "
+              f"{r['L2_mixed'][0]:.3e}; mixed vs FP64 max {max(r['mixed_vs_fp64'].values()):.2e}; "
+              f"div B {r['divB_fp64']:.1e} / {r['divB_mixed']:.1e}")
+    assert rows[1]["order_fp64"][0] > 2.9 and rows[2]["order_fp64"][0] > 2.9
+    assert all(max(r["mixed_vs_fp64"].values()) < 1e-5 for r in rows)
+    assert all(r["divB_fp64"] < 1e-12 and r["divB_mixed"] < 1e-12 for r in rows)
+
+
 def test_plain_1024_vs_oracle(pkg, oracle_mod):
     """the plain method's own configuration at its size: perturbed
     Orszag-Tang 1024^2, HLLD, TVB (M = 50) gated by KXRCF, positivity, CFL
