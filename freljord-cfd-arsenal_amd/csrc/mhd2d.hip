@@ -1392,7 +1392,10 @@ int step_body(mhd2d_ctx* ctx) {
     const hipStream_t s1 = ctx->stream, s2 = ctx->stream2;
     if (fused && !mixed) {
       // vertex EMFs beside the face kernels; the cell kernel's limiter
-      // epilogue reads the new faces, so it follows the CT
+      // epilogue reads the new faces, so it follows the CT.  As in the mixed
+      // form below, the stage value's planes other than Bx, By live in lim
+      // (its Bx, By planes in out) until the list kernels limit cells in
+      // place, so the detection reads lim and out and runs first.
       if ((r = wait(ctx->ev_fork, s1, s2))) return r;
       if ((r = on2(vertex))) return r;
       if ((r = launch_fluxes(ctx, ctx->S, false, sc))) return r;
@@ -1400,7 +1403,9 @@ int step_body(mhd2d_ctx* ctx) {
       if ((r = on2(ct))) return r;
       if ((r = wait(ctx->ev_join, s2, s1))) return r;
       if ((r = launch_cell(ctx, k, ctx->S, A, ctx->U0, out, sc, 0.0, lim, Fo, pp))) return r;
-      return launch_limit_fixup(ctx, out, lim, Fo, pp, sc, k);
+      if ((r = launch_detection(ctx, lim, sc, k, out))) return r;
+      if (pp && (r = launch_pp_list(ctx, out, lim, Fo, k))) return r;
+      return launch_limit_relist(ctx, out, lim, Fo, pp, k, true);
     }
     if (fused) {
       // mixed form: vertex EMFs beside the face kernels, then the CT update

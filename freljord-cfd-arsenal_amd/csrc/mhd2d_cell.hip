@@ -33,8 +33,11 @@ namespace {
 // scaled by 1/hx1, 1/hy1 point by point; the volume sums are scaled once
 // LIM (plain form): the epilogue also applies the limiter pass without TVB
 // (positivity when PP, then the divergence-free reconstruction from the
-// stage's new faces Fn) to the stage value as stored, and writes the result
-// to lout; the cells TVB changes are limited again afterwards (k_limit_list).
+// stage's new faces Fn) to the stage value as stored.  The stage value's
+// planes other than Bx, By go once into lout (the limited value of every cell
+// positivity leaves alone), its Bx, By planes into out, the limited Bx, By
+// planes into lout; the cells positivity changes go to pplist and are limited
+// in place afterwards (k_pp_list), as are the cells TVB changes (k_limit_list).
 // This saves the limiter pass's re-read of the stage value and its faces.
 // In the mixed form the epilogue is lean: a full positivity epilogue needs
 // 254 VGPRs, which halves the kernel's waves per SIMD and costs more than
@@ -234,6 +237,7 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(
   }
   // epilogue: mass inverse + RK combination, in double
   constexpr bool LEAN = LIM && MIX;   // the mixed form's lean limiter epilogue
+  constexpr bool SHARE = LIM;         // the stage value shares its planes other than Bx, By with lout
   double m[LIM && !LEAN ? NPL : 1];   // LIM: the stage value as stored (FP32 planes rounded)
   double pa[LEAN ? 8 : 1], pav[LEAN ? 8 : 1];   // LEAN: positivity bound sums and the averages
 #pragma unroll
@@ -248,12 +252,12 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(
   // planes) and a re-read of the FP64 cell averages.  An aq lane (rare)
   // refills its LDS column with A's planes first, so that one code path
   // serves both.
-  if constexpr (LEAN && MODE >= 2) {
+  if constexpr (SHARE && MODE >= 2) {
     const int tag = (int)(4 * sc->epoch + (MODE - 1));
     if (interior && lflag[(long long)(j - 1) * g.Nx + (i - 1)] == tag) {
       static_for<0, NPL>([&](auto P) {
         constexpr int p = P;
-        if constexpr (!is_bplane(p) && !pl_f64(p)) {
+        if constexpr (!is_bplane(p) && (sizeof(T) == 8 || !pl_f64(p))) {
           const T v = T(A(p, c));
           sm[lds_slot(p) * CBS + tid] = v;   // no register plane holds one of these
         }
@@ -268,17 +272,20 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(
     // the LDS copy is exact for the planes stored in the quadrature type
     auto s_exact = [&]() -> double {
       return interior && (sizeof(T) == 8 || !pl_f64(p)) ? (double)(in_rg(p) ? rg[in_rg(p) ? p - R0 : 0] : sm[lds_slot(p) * CBS + tid])
-                                                         : ((LEAN && MODE >= 2 && !interior) ? A(p, c) : S(p, c));
+                                                         : ((SHARE && MODE >= 2 && !interior) ? A(p, c) : S(p, c));
     };
     double a = 0.0;
-    if (MODE >= 2) a = (LEAN && !is_bplane(p)) ? s_exact() : A(p, c);
+    if (MODE >= 2) a = (SHARE && !is_bplane(p)) ? s_exact() : A(p, c);
     double r;
     if (MODE == 0) r = du;
     else if (MODE == 1) r = s_exact() + dt * du;
     else if (MODE == 2) r = (3.0 / 4.0) * U0(p, c) + (1.0 / 4.0) * (a + dt * du);
     else r = (1.0 / 3.0) * U0(p, c) + (2.0 / 3.0) * (a + dt * du);
-    if (!LEAN || is_bplane(p) || !interior) out.set(p, c, r);
-    if constexpr (LIM && !LEAN) m[p] = (MIX && !pl_f64(p)) ? (double)(float)r : r;
+    if (!SHARE || is_bplane(p) || !interior) out.set(p, c, r);
+    if constexpr (LIM && !LEAN) {
+      m[p] = r;
+      if (!is_bplane(p)) lout.set(p, c, r);   // the stage value, also the limited value of a cell pp leaves alone
+    }
     if constexpr (LEAN) {
       const double v = pl_f64(p) ? r : (double)(float)r;   // the value as stored
       if (!is_bplane(p)) lout.set(p, c, v);                  // the stage value, also the limited value of a settled cell
@@ -301,16 +308,23 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(
       pp_append(need, interior ? (j - 1) * g.Nx + (i - 1) : 0, pplist, lst);
     }
   } else if constexpr (LIM) {
-    // the limiter pass of k_limit<false, PP, true> on the stored stage value
+    // the limiter pass of k_limit<false, PP, true> on the stored stage value:
+    // the cells positivity changes go to pplist (k_pp_list limits them in
+    // place after the detection has read the stage value); the limited Bx, By
+    // planes come from the faces (div_free_Balsara)
+    bool changed = false;
     if (interior) {
-      if (PP) pp_cell(m);
+      if (PP) changed = pp_cell(m);
       divfree_cell(m, Fn, g, i, j);
 #pragma unroll
-      for (int p = 0; p < NPL; p++) lout.set(p, c, m[p]);
+      for (int p = 0; p < NPL; p++)
+        if (is_bplane(p)) lout.set(p, c, m[p]);
     } else {
 #pragma unroll
-      for (int p = 0; p < NPL; p++) lout.set(p, c, is_bplane(p) ? 0.0 : m[p]);
+      for (int p = 0; p < NPL; p++)
+        if (is_bplane(p)) lout.set(p, c, 0.0);
     }
+    if constexpr (PP) pp_append(changed, interior ? (j - 1) * g.Nx + (i - 1) : 0, pplist, lst);
   }
 }
 
@@ -469,10 +483,12 @@ void launch_cell_t(mhd2d_ctx* ctx, const double* S, const double* A, const doubl
                            ctx->sc, (const int*)ctx->lflag);
       return;
     }
+    hipLaunchKernelGGL(k_reset_npp, dim3(1), dim3(64), 0, ctx->stream, ctx->sc, g.gate);
     hipLaunchKernelGGL((k_cell<MODE, double, CBS_F64, false, false, true, PP>),
                        dim3((unsigned)((n + CBS_F64 - 1) / CBS_F64)), dim3(CBS_F64), 0, ctx->stream, SRd<false>{S, pl},
                        (const double*)ctx->FX, (const double*)ctx->FY, SRd<false>{A, pl}, SRd<false>{U0, pl},
-                       SWr<false>{out, pl}, SWr<false>{lout, pl}, Fn, g, sc, dt);
+                       SWr<false>{out, pl}, SWr<false>{lout, pl}, Fn, g, sc, dt, ctx->pplist, ctx->sc,
+                       (const int*)ctx->lflag);
     return;
   }
   if (ctx->cfg.precision == MHD2D_MIXED && g.hx1 == g.hy1)

@@ -290,20 +290,20 @@ int launch_candidates_tvb(mhd2d_ctx* ctx, const double* in, DevScal* sc, const d
 // stage 1, and reuses the screen for stages 2 and 3); 0 for a standalone call
 int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool tvb, bool pp, bool div,
                  DevScal* sc, int stage = 0);                                                                // mhd2d_limit.hip
-// after a limiter pass without TVB into out (the plain form's fused cell
-// epilogue, or k_limit<false, ..>): detection and TVB on the unlimited stage
-// value `in` (KXRCF-gated and learned indicators), then the cells TVB changed
-// limited again from `in` into out.  launch_limit_fixup = both.
+// after a limiter pass without TVB into out (the fused cell epilogue, or
+// k_limit<false, ..>): detection and TVB on the unlimited stage value `in`
+// (KXRCF-gated and learned indicators; inB: a second buffer holding its Bx,
+// By planes), then the cells TVB changed limited again from `in` into out
+// (lean: in place, out holding the stage value's other planes and `in` its
+// Bx, By planes, the fused stages of both forms).
 int launch_detection(mhd2d_ctx* ctx, const double* in, DevScal* sc, int stage, const double* inB = nullptr);                    // mhd2d_limit.hip
 int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, int stage,
                         bool lean = false);  // mhd2d_limit.hip
-// the cells the mixed cell kernel's epilogue left to the positivity limiter
+// the cells the fused cell kernel's epilogue left to the positivity limiter
 // (the list of its bound test), limited from `in` into out
 int launch_pp_list(mhd2d_ctx* ctx, double* q, double* out, const double* F, int stage);                 // mhd2d_limit.hip
 // div_free_Balsara into the B planes of out from faces F (halo B zeroed);
 // the other planes are left as they are
 int launch_divfree(mhd2d_ctx* ctx, double* out, const double* F);                                  // mhd2d_limit.hip
-int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, DevScal* sc,
-                       int stage);                                                                           // mhd2d_limit.hip
 
 }  // namespace mhd2d_impl

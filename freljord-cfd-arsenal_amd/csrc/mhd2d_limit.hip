@@ -420,7 +420,6 @@ int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const dou
                         bool lean) {
   const int ind = ctx->cfg.indicator;
   if (ind != MHD2D_IND_MLP && ind != MHD2D_IND_KXRCF) return 0;
-  if (lean && ctx->cfg.precision != MHD2D_MIXED) return fail(ctx, -1, "launch_limit_relist: lean stage in FP64");
   const Geo& g = ctx->g;
   Timed tm(ctx, KC_LIMIT);
   const unsigned nb = std::min<unsigned>(nblk((long long)g.Nx * g.Ny), 1024u);
@@ -429,7 +428,7 @@ int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const dou
     const SRd<MIX> vi{in, g.plane};
     const SWr<MIX> vo{out, g.plane};
     auto go = [&](auto Pc, auto Lc) {
-      constexpr bool P = decltype(Pc)::value, L = decltype(Lc)::value && MIX;
+      constexpr bool P = decltype(Pc)::value, L = decltype(Lc)::value;
       hipLaunchKernelGGL((k_limit_list<P, MIX, L>), dim3(nb), dim3(BS), 0, ctx->stream, vi, vo, F,
                          (const int*)ctx->tlist, (const int*)ctx->tmask, (const double*)ctx->trec, g,
                          (const DevScal*)ctx->sc, ctx->lflag, stage);
@@ -446,15 +445,17 @@ int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const dou
 }
 
 int launch_pp_list(mhd2d_ctx* ctx, double* q, double* out, const double* F, int stage) {
-  if (ctx->cfg.precision != MHD2D_MIXED) return 0;
   const Geo& g = ctx->g;
   Timed tm(ctx, KC_LIMIT);
   const unsigned nb = std::min<unsigned>(nblk((long long)g.Nx * g.Ny), 1024u);
-  hipLaunchKernelGGL((k_pp_list<true>), dim3(nb), dim3(BS), 0, ctx->stream, SWr<true>{q, g.plane},
-                     SWr<true>{out, g.plane}, F, (const int*)ctx->pplist, g, (const DevScal*)ctx->sc, ctx->lflag,
-                     stage);
-  LAUNCHCHK(ctx);
-  return 0;
+  return with_mix(ctx, [&](auto Mc) -> int {
+    constexpr bool MIX = decltype(Mc)::value;
+    hipLaunchKernelGGL((k_pp_list<MIX>), dim3(nb), dim3(BS), 0, ctx->stream, SWr<MIX>{q, g.plane},
+                       SWr<MIX>{out, g.plane}, F, (const int*)ctx->pplist, g, (const DevScal*)ctx->sc, ctx->lflag,
+                       stage);
+    LAUNCHCHK(ctx);
+    return 0;
+  });
 }
 
 int launch_divfree(mhd2d_ctx* ctx, double* out, const double* F) {
@@ -469,13 +470,5 @@ int launch_divfree(mhd2d_ctx* ctx, double* out, const double* F) {
   });
 }
 
-int launch_limit_fixup(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, DevScal* sc,
-                       int stage) {
-  const int ind = ctx->cfg.indicator;
-  if (ind == MHD2D_IND_NONE) return 0;
-  if (ind == MHD2D_IND_ALL) return fail(ctx, -1, "launch_limit_fixup: TVB on every cell has no fused limiter");
-  int rc = launch_detection(ctx, in, sc, stage);
-  return rc ? rc : launch_limit_relist(ctx, in, out, F, pp, stage);
-}
 
 }  // namespace mhd2d_impl

@@ -73,9 +73,12 @@ __device__ __forceinline__ bool pp_settled(const double m[NPL], double eps) {
   return pp_settled_sums(a, avg, eps);
 }
 
-__device__ __forceinline__ void pp_cell(double m[NPL]) {
+// returns whether the limiter changed the cell (a scaling by eta = 1 leaves
+// every value's bits as they were)
+__device__ __forceinline__ bool pp_cell(double m[NPL]) {
   const double eps = 1e-13;
-  if (pp_settled(m, eps)) return;
+  if (pp_settled(m, eps)) return false;
+  bool changed = false;
   // density: eta = the LAST |(rbar - eps)/(rbar - rq)| below 1.  For
   // doubles, RN(|a/b|) < 1 exactly when |a| < |b|, so only that point's
   // quotient is formed.
@@ -95,6 +98,7 @@ __device__ __forceinline__ void pp_cell(double m[NPL]) {
       }
   if (any) eta = fabs(na / nb);
   if (eta < 1.0) {
+    changed = true;
 #pragma unroll
     for (int b = 1; b < 6; b++) m[PL(b, 0)] = 0.5 * eta * m[PL(b, 0)];
   }
@@ -170,7 +174,10 @@ __device__ __forceinline__ void pp_cell(double m[NPL]) {
     const double eta1 = sqrt(s1) / sqrt(s2);
     if (eta1 < eta) eta = eta1;
   }
-  if (eta < 1.0) eta = 0.5 * eta;
+  if (eta < 1.0) {
+    eta = 0.5 * eta;
+    changed = true;
+  }
 #pragma unroll
   for (int n = 0; n < 8; n++)
 #pragma unroll
@@ -178,6 +185,7 @@ __device__ __forceinline__ void pp_cell(double m[NPL]) {
       const int p = PL(b, n);
       if (p >= 0) m[p] = eta * m[p];
     }
+  return changed;
 }
 
 // div_free_Balsara.f90:15-78: in-cell B from the four faces; set(p, v)
