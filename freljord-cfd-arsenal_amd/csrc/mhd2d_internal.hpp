@@ -297,7 +297,7 @@ int launch_limit(mhd2d_ctx* ctx, const double* in, double* out, const double* F,
 // (lean: in place, out holding the stage value's other planes and `in` its
 // Bx, By planes, the fused stages of both forms).
 int launch_detection(mhd2d_ctx* ctx, const double* in, DevScal* sc, int stage, const double* inB = nullptr);                    // mhd2d_limit.hip
-int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, int stage,
+int launch_limit_relist(mhd2d_ctx* ctx, double* in, double* out, const double* F, bool pp, int stage,
                         bool lean = false);  // mhd2d_limit.hip
 // the cells the fused cell kernel's epilogue left to the positivity limiter
 // (the list of its bound test), limited from `in` into out

@@ -92,7 +92,7 @@ __device__ __forceinline__ void lean_take(const SWr<MIX>& q, const SWr<MIX>& out
 // unlimited stage value, with TVB (a fixed grid strides the list).  LEAN:
 // the mixed form's lean stage (in is the companion, lean_take)
 template <bool PP, bool MIX, bool LEAN>
-__global__ __launch_bounds__(BS) void k_limit_list(SRd<MIX> in, SWr<MIX> out, const double* __restrict__ F,
+__global__ __launch_bounds__(BS) void k_limit_list(SWr<MIX> in, SWr<MIX> out, const double* __restrict__ F,
                                                    const int* __restrict__ list, const int* __restrict__ tmask,
                                                    const double* __restrict__ trec, Geo g,
                                                    const DevScal* __restrict__ lst, int* __restrict__ lflag,
@@ -103,8 +103,8 @@ __global__ __launch_bounds__(BS) void k_limit_list(SRd<MIX> in, SWr<MIX> out, co
   for (unsigned int k = blockIdx.x * BS + threadIdx.x; k < n; k += gridDim.x * BS) {
     const int tc = list[k];
     if (!tmask[tc]) continue;
-    if (LEAN) lean_take(SWr<MIX>{const_cast<double*>(in.b), in.pl}, out, lflag, tag, g, tc);
-    limit_cell<true, PP, true, MIX>(in, out, F, tmask, trec, g, 1 + tc % g.Nx, 1 + tc / g.Nx);
+    if (LEAN) lean_take(in, out, lflag, tag, g, tc);
+    limit_cell<true, PP, true, MIX>(SRd<MIX>(in), out, F, tmask, trec, g, 1 + tc % g.Nx, 1 + tc / g.Nx);
   }
 }
 
@@ -416,7 +416,7 @@ int launch_detection(mhd2d_ctx* ctx, const double* in, DevScal* sc, int stage, c
   return launch_detect_tvb(ctx, in, sc, inB);
 }
 
-int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const double* F, bool pp, int stage,
+int launch_limit_relist(mhd2d_ctx* ctx, double* in, double* out, const double* F, bool pp, int stage,
                         bool lean) {
   const int ind = ctx->cfg.indicator;
   if (ind != MHD2D_IND_MLP && ind != MHD2D_IND_KXRCF) return 0;
@@ -425,8 +425,7 @@ int launch_limit_relist(mhd2d_ctx* ctx, const double* in, double* out, const dou
   const unsigned nb = std::min<unsigned>(nblk((long long)g.Nx * g.Ny), 1024u);
   return with_mix(ctx, [&](auto Mc) -> int {
     constexpr bool MIX = decltype(Mc)::value;
-    const SRd<MIX> vi{in, g.plane};
-    const SWr<MIX> vo{out, g.plane};
+    const SWr<MIX> vi{in, g.plane}, vo{out, g.plane};
     auto go = [&](auto Pc, auto Lc) {
       constexpr bool P = decltype(Pc)::value, L = decltype(Lc)::value;
       hipLaunchKernelGGL((k_limit_list<P, MIX, L>), dim3(nb), dim3(BS), 0, ctx->stream, vi, vo, F,
