@@ -190,31 +190,42 @@ __device__ __forceinline__ bool pp_cell(double m[NPL]) {
 
 // div_free_Balsara.f90:15-78: in-cell B from the four faces; set(p, v)
 // receives plane p of the reconstruction
-template <class Set>
-__device__ __forceinline__ void divfree_apply(Set&& set, const double* __restrict__ F, const Geo& g, int i, int j) {
+// div_free_Balsara.f90:15-78 for one Bx or By plane p of interior cell (i,
+// j), from its four faces (Bx modes on the x faces i-1/2, i+1/2, By modes on
+// the y faces j-1/2, j+1/2).  Contraction is off, so every kernel that
+// reconstructs (the limiter passes, the fused cell epilogue, k_divfree)
+// gets the same bits, each operation rounded as the reference rounds it.
+// p is a constant wherever this is inlined, so the choice below folds away.
+__device__ __forceinline__ double divfree_plane(int p, const double* __restrict__ F, const Geo& g, int i, int j) {
+  MHD_NOCONTRACT
   const long long pl = g.plane;
   const long long cR = cix(g, i, j), cL = cix(g, i - 1, j), cD = cix(g, i, j - 1);
-  const double a0R = F[0 * pl + cR], a1R = F[1 * pl + cR], a2R = F[2 * pl + cR];
-  const double a0L = F[0 * pl + cL], a1L = F[1 * pl + cL], a2L = F[2 * pl + cL];
-  const double b0U = F[3 * pl + cR], b1U = F[4 * pl + cR], b2U = F[5 * pl + cR];
-  const double b0D = F[3 * pl + cD], b1D = F[4 * pl + cD], b2D = F[5 * pl + cD];
+  auto a = [&](int k, long long c) { return F[k * pl + c]; };         // Bx mode k + 1 on an x face
+  auto b = [&](int k, long long c) { return F[(3 + k) * pl + c]; };   // By mode k + 1 on a y face
   const double rxy = g.hx / g.hy, ryx = g.hy / g.hx;
-  set(PL(0, 5), 0.5 * (a0R + a0L) + (1.0 / 6.0) * rxy * (b1U - b1D));
-  set(PL(1, 5), 0.5 * (a0R - a0L) + (1.0 / 15.0) * rxy * (b2U - b2D));
-  set(PL(2, 5), 0.5 * (a1R + a1L));
-  set(PL(3, 5), -(1.0 / 4.0) * rxy * (b1U - b1D));
-  set(PL(4, 5), 0.5 * (a1R - a1L));
-  set(PL(5, 5), 0.5 * (a2R + a2L));
-  set(PL(6, 5), -(1.0 / 6.0) * rxy * (b2U - b2D));
-  set(PL(8, 5), 0.5 * (a2R - a2L));
-  set(PL(0, 6), 0.5 * (b0U + b0D) + (1.0 / 6.0) * ryx * (a1R - a1L));
-  set(PL(2, 6), 0.5 * (b0U - b0D) + (1.0 / 15.0) * ryx * (a2R - a2L));
-  set(PL(1, 6), 0.5 * (b1U + b1D));
-  set(PL(5, 6), -(1.0 / 4.0) * ryx * (a1R - a1L));
-  set(PL(4, 6), 0.5 * (b1U - b1D));
-  set(PL(3, 6), 0.5 * (b2U + b2D));
-  set(PL(9, 6), -(1.0 / 6.0) * ryx * (a2R - a2L));
-  set(PL(7, 6), 0.5 * (b2U - b2D));
+  if (p == PL(0, 5)) return 0.5 * (a(0, cR) + a(0, cL)) + (1.0 / 6.0) * rxy * (b(1, cR) - b(1, cD));
+  if (p == PL(1, 5)) return 0.5 * (a(0, cR) - a(0, cL)) + (1.0 / 15.0) * rxy * (b(2, cR) - b(2, cD));
+  if (p == PL(2, 5)) return 0.5 * (a(1, cR) + a(1, cL));
+  if (p == PL(3, 5)) return -(1.0 / 4.0) * rxy * (b(1, cR) - b(1, cD));
+  if (p == PL(4, 5)) return 0.5 * (a(1, cR) - a(1, cL));
+  if (p == PL(5, 5)) return 0.5 * (a(2, cR) + a(2, cL));
+  if (p == PL(6, 5)) return -(1.0 / 6.0) * rxy * (b(2, cR) - b(2, cD));
+  if (p == PL(8, 5)) return 0.5 * (a(2, cR) - a(2, cL));
+  if (p == PL(0, 6)) return 0.5 * (b(0, cR) + b(0, cD)) + (1.0 / 6.0) * ryx * (a(1, cR) - a(1, cL));
+  if (p == PL(2, 6)) return 0.5 * (b(0, cR) - b(0, cD)) + (1.0 / 15.0) * ryx * (a(2, cR) - a(2, cL));
+  if (p == PL(1, 6)) return 0.5 * (b(1, cR) + b(1, cD));
+  if (p == PL(5, 6)) return -(1.0 / 4.0) * ryx * (a(1, cR) - a(1, cL));
+  if (p == PL(4, 6)) return 0.5 * (b(1, cR) - b(1, cD));
+  if (p == PL(3, 6)) return 0.5 * (b(2, cR) + b(2, cD));
+  if (p == PL(9, 6)) return -(1.0 / 6.0) * ryx * (a(2, cR) - a(2, cL));
+  return 0.5 * (b(2, cR) - b(2, cD));                                   // PL(7, 6)
+}
+template <class Set>
+__device__ __forceinline__ void divfree_apply(Set&& set, const double* __restrict__ F, const Geo& g, int i, int j) {
+  constexpr int planes[16] = {PL(0, 5), PL(1, 5), PL(2, 5), PL(3, 5), PL(4, 5), PL(5, 5), PL(6, 5), PL(8, 5),
+                              PL(0, 6), PL(2, 6), PL(1, 6), PL(5, 6), PL(4, 6), PL(3, 6), PL(9, 6), PL(7, 6)};
+#pragma unroll
+  for (int k = 0; k < 16; k++) set(planes[k], divfree_plane(planes[k], F, g, i, j));
 }
 __device__ __forceinline__ void divfree_cell(double m[NPL], const double* __restrict__ F, const Geo& g, int i, int j) {
   divfree_apply([&](int p, double v) { m[p] = v; }, F, g, i, j);
