@@ -35,11 +35,11 @@ struct DevScal {
   long long nflag;
   unsigned int nlist;   // length of the troubled-cell list of the current limiter pass
   unsigned int nband;   // length of the indicator's candidate list (cells handed to KXRCF)
-  unsigned int npp;     // length of the list of cells the positivity limiter must look at (mixed form)
+  unsigned int npp;     // length of the list of cells the positivity limiter must look at (fused stages)
   int active;
   int health;           // MHD2D_FLAG_* bits, sticky until mhd2d_set_time
   long long epoch;      // passes that advanced, since the context was created (never reset): the
-                        // mixed form's limiter lists mark cells with 4 epoch + stage (lflag)
+                        // fused stages' limiter lists mark cells with 4 epoch + stage (lflag)
 };
 
 __device__ __forceinline__ long long dkey(double x) {   // order-preserving key of a double
@@ -168,8 +168,8 @@ struct mhd2d_ctx {
   int* tlist;          // interior-cell indices the detector flagged (length sc->nlist)
   int* blist;          // interior-cell indices the indicator hands to KXRCF (sc->nband)
   int* screen;         // indicator: 1 where p >= mlp_lo (before the one-cell dilation)
-  int* pplist;         // mixed form: interior cells whose positivity bound does not settle them (sc->npp)
-  int* lflag;          // mixed form: 4 epoch + stage where a list kernel limited the cell in place
+  int* pplist;         // fused stages: interior cells the cell kernel's epilogue leaves to positivity (sc->npp)
+  int* lflag;          // fused stages: 4 epoch + stage where a list kernel limited the cell in place
   double* trec;        // modified edge values of changed cells: 32 planes over the interior cells
   void* mlpw;          // indicator MLP weights in MFMA operand order (mhd2d_mlp.hip)
   float* prob;         // indicator MLP probability per interior cell

@@ -68,10 +68,10 @@ __global__ __launch_bounds__(BS) void k_limit(SRd<MIX> in, SWr<MIX> out,
   limit_cell<APPLY, PP, DIV, MIX>(in, out, F, tmask, trec, g, (int)(t % W), (int)(t / W));
 }
 
-// The mixed form's lean stage stores the stage value's planes other than
-// Bx, By once, in the limited value (out); its companion (q) holds the Bx, By
-// planes.  Before a list kernel limits interior cell tc in place, the cell's
-// other unlimited planes are copied to the companion and the cell is marked
+// The fused stages store the stage value's planes other than Bx, By once,
+// in the limited value (out); its companion (q) holds the Bx, By planes.
+// Before a list kernel limits interior cell tc in place, the cell's other
+// unlimited planes are copied to the companion and the cell is marked
 // with this stage's tag 4 epoch + stage (the next stage's cell kernel then
 // reads its RK input there); the companion then holds the whole unlimited
 // value of the cell.
@@ -90,7 +90,7 @@ __device__ __forceinline__ void lean_take(const SWr<MIX>& q, const SWr<MIX>& out
 // kernel's fused epilogue limited every cell without TVB (pp and the
 // divergence-free reconstruction), so these are limited again from the
 // unlimited stage value, with TVB (a fixed grid strides the list).  LEAN:
-// the mixed form's lean stage (in is the companion, lean_take)
+// the fused stages' shared planes (in is the companion, lean_take)
 template <bool PP, bool MIX, bool LEAN>
 __global__ __launch_bounds__(BS) void k_limit_list(SWr<MIX> in, SWr<MIX> out, const double* __restrict__ F,
                                                    const int* __restrict__ list, const int* __restrict__ tmask,
@@ -108,11 +108,12 @@ __global__ __launch_bounds__(BS) void k_limit_list(SWr<MIX> in, SWr<MIX> out, co
   }
 }
 
-// the cells the mixed cell kernel's epilogue could not settle with the
-// positivity bound: the full limiter pass without TVB (positivity, then the
-// divergence-free reconstruction) from the unlimited stage value (in, the
-// companion, after lean_take); the cells TVB changes are limited again
-// afterwards (k_limit_list)
+// the cells the fused cell kernel's epilogue left to the positivity limiter
+// (mixed form: those its bound cannot settle; plain form: those positivity
+// changes): the full limiter pass without TVB (positivity, then the
+// divergence-free reconstruction) from the unlimited stage value (the
+// companion q, after lean_take), in place; the cells TVB changes are limited
+// again afterwards (k_limit_list)
 template <bool MIX>
 __global__ __launch_bounds__(BS) void k_pp_list(SWr<MIX> q, SWr<MIX> out, const double* __restrict__ F,
                                                 const int* __restrict__ list, Geo g,

@@ -224,7 +224,7 @@ struct SRd {
   }
 };
 // the same over two buffers of that layout: the Bx, By planes from bq (when
-// set), the others from b.  The mixed form's unlimited stage value is read
+// set), the others from b.  The fused stages' unlimited stage value is read
 // so: its planes other than Bx, By share the limited value's storage.
 template <bool MIX>
 struct SRdB {
