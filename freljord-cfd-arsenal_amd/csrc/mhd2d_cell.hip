@@ -164,7 +164,7 @@ __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(
     if constexpr (MIX) {
       static_for<0, 5>(row);
     } else {
-#pragma unroll
+#pragma unroll 1
       for (int j1 = 0; j1 < 5; j1++) row(j1);
     }
     if (SQ) {
