@@ -372,16 +372,19 @@ def test_invariants_2048(pkg):
     assert d["pmin"] > 0
 
 
-@pytest.mark.parametrize("N,steps", [(256, 20)])
-def test_mixed_vs_fp64(pkg, N, steps):
+@pytest.mark.parametrize("N,steps,ft", [(256, 20, 4), (128, 20, 1), (128, 20, 2), (128, 20, 3)],
+                         ids=["HLLD", "LF", "HLL", "HLLC"])
+def test_mixed_vs_fp64(pkg, N, steps, ft):
     """mixed form (FP32 face fluxes, quadratures and higher-mode storage;
     FP64 averages, magnetic modes, CT, div-free, limiters) against the plain FP64 form on the same input:
-    relative L2 of every field <= 1e-5 (density is the headline), div B at
-    roundoff"""
+    relative L2 of every field <= 1e-6 (measured: 1.9-2.4e-7 for the four
+    flux types), div B at roundoff.  Every flux type: the mixed face kernels
+    and the vertex EMF are instantiated per flux type (the plain form's flux
+    types are pinned to the oracle by test_rk3_fused and test_scale_vs_oracle)."""
     dom = (0.0, 2 * np.pi, 0.0, 2 * np.pi)
     out = {}
     for prec in (pkg.FP64, pkg.MIXED):
-        s = pkg.Solver(N, N, flux_type=pkg.HLLD, domain=dom, M=50.0, CFL=0.2, pp=1, precision=prec)
+        s = pkg.Solver(N, N, flux_type=ft, domain=dom, M=50.0, CFL=0.2, pp=1, precision=prec)
         s.init_data(pkg.IC_OT_PERTURBED, [1e-2, 20261203])
         s.rk3_prologue()
         for _ in range(steps):
@@ -390,10 +393,11 @@ def test_mixed_vs_fp64(pkg, N, steps):
     (u64, _, _), d64, t64 = out[pkg.FP64]
     (u32, _, _), d32, t32 = out[pkg.MIXED]
     assert abs(t64[0] - t32[0]) < 1e-6 * t64[0]
-    for n in range(8):
-        a, b = interior(u32)[:, :, :, n], interior(u64)[:, :, :, n]
-        if np.linalg.norm(b) > 0:
-            assert rel_l2(a, b) < 1e-5, n
+    errs = [rel_l2(interior(u32)[:, :, :, n], interior(u64)[:, :, :, n]) for n in range(8)
+            if np.linalg.norm(interior(u64)[:, :, :, n]) > 0]
+    print(f"[mixed] flux type {ft}, {N}^2, {steps} steps: largest field rel L2 mixed vs FP64 {max(errs):.2e}, "
+          f"totaldiv {d32['totaldiv']:.1e}")
+    assert max(errs) < 1e-6
     assert d32["totaldiv"] < 1e-12
 
 
@@ -429,7 +433,7 @@ def test_mixed_shock_rotor(pkg):
     close to the plain form (measured 6.1e-6 in density after 20 steps at
     128^2; the bar is about three times that).  FP32 rounding can move a
     limiter decision at a discontinuity, so the bar is looser than the smooth
-    benchmark flow's 1e-5 (test_mixed_vs_fp64).  The mixed form's list
+    benchmark flow's 1e-6 (test_mixed_vs_fp64).  The mixed form's list
     kernels limit the cells TVB changes in place and keep their unlimited
     value apart for the next stage's RK combination; combining the limited
     value instead would move every such cell by a share of the limiter's
