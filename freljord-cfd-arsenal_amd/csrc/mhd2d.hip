@@ -1358,13 +1358,15 @@ int step_body(mhd2d_ctx* ctx) {
   // cell kernel's epilogue (the CT goes first so the new faces are there),
   // and only the cells TVB changes are limited again; in the mixed form the
   // epilogue leaves the cells its positivity bound cannot settle to a list
-  // kernel, which runs beside the detection, and the limited B planes to a
-  // pass after the CT that runs beside the cell kernel.
+  // kernel, and the limited B planes to a pass after the CT update, which
+  // needs no CT before it: the CT update and that pass run after the cell
+  // kernel, beside the detection.
   const bool mixed = ctx->cfg.precision == MHD2D_MIXED;
   const bool fused = ctx->cfg.indicator != MHD2D_IND_ALL;
-  // The vertex EMFs (FP64 VALU-bound) and then the CT update run on a second
-  // stream beside the face kernels (and, unfused, the cell kernel), forked
-  // from and joined back into the context stream by events inside each stage.
+  // The vertex EMFs (FP64 VALU-bound) run on a second stream beside the face
+  // kernels and the CT update after them on that stream (beside the cell
+  // kernel unfused, beside the detection in the mixed form), forked from and
+  // joined back into the context stream by events inside each stage.
   // With the per-kernel event timing on, everything stays on one stream so
   // that each kernel is timed alone.  Graph capture also stays on one stream:
   // with this ROCm, a captured graph with parallel branches keeps the buffers
@@ -1408,27 +1410,29 @@ int step_body(mhd2d_ctx* ctx) {
       return launch_limit_relist(ctx, out, lim, Fo, pp, k, true);
     }
     if (fused) {
-      // mixed form: vertex EMFs beside the face kernels, then the CT update
-      // and the divergence-free B planes of the limited value beside the
-      // cell kernel (its lean epilogue writes the other planes and needs no
-      // faces).  The stage value's planes other than Bx, By live in lim
-      // (its Bx, By planes in out) until the list kernels limit cells in
-      // place, so the detection reads lim and out and runs first.
+      // mixed form: vertex EMFs beside the face kernels; the cell kernel's
+      // lean epilogue writes the planes other than Bx, By and needs no
+      // faces, so the CT update and the divergence-free B planes of the
+      // limited value follow it, beside the detection (memory-bound work
+      // beside the memory-bound cell kernel only stretched it: 8.29-8.30
+      // against 8.11 ms per step at 2048^2, measured).  The stage value's
+      // planes other than Bx, By live in lim (its Bx, By planes in out)
+      // until the list kernels limit cells in place, so the detection reads
+      // lim and out and runs first.
       if ((r = wait(ctx->ev_fork, s1, s2))) return r;
       if ((r = on2(vertex))) return r;
       if ((r = launch_fluxes(ctx, ctx->S, false, sc))) return r;
-      if ((r = wait(ctx->ev_faces, s1, s2))) return r;
+      if ((r = launch_cell(ctx, k, ctx->S, A, ctx->U0, out, sc, 0.0, lim, Fo, pp))) return r;
+      // the CT update and the limited B planes (memory-bound, as the cell
+      // kernel is) on the second stream after the cell kernel, beside the
+      // detection chain; then the positivity list and the cells TVB changed,
+      // limited again (after the list, so that their TVB-limited value is
+      // the one that stays)
+      if ((r = wait(ctx->ev_cell, s1, s2))) return r;
       if ((r = on2(ct)) || (r = on2([&] { return launch_divfree(ctx, lim, Fo); }))) return r;
       if (two) HIPCHK(ctx, hipEventRecord(ctx->ev_join, s2));       // new faces and limited B done
-      if ((r = launch_cell(ctx, k, ctx->S, A, ctx->U0, out, sc, 0.0, lim, Fo, pp))) return r;
-      // detection and TVB on the second stream, then the positivity list and
-      // the cells TVB changed, limited again (after the list, so that their
-      // TVB-limited value is the one that stays)
-      if ((r = wait(ctx->ev_cell, s1, s2))) return r;
-      if ((r = on2([&] { return launch_detection(ctx, lim, sc, k, out); }))) return r;
-      if (two) HIPCHK(ctx, hipEventRecord(ctx->ev_det, s2));
+      if ((r = launch_detection(ctx, lim, sc, k, out))) return r;
       if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_join, 0));
-      if (two) HIPCHK(ctx, hipStreamWaitEvent(s1, ctx->ev_det, 0));
       if (pp && (r = launch_pp_list(ctx, out, lim, Fo, k))) return r;
       return launch_limit_relist(ctx, out, lim, Fo, pp, k, true);
     }

@@ -49,8 +49,8 @@ namespace {
 // are appended to pplist and limited in place afterwards by a list kernel
 // (k_pp_list), which keeps their unlimited planes in out.  The limited B
 // planes come from the faces alone (div_free_Balsara), so a separate pass
-// (k_divfree) writes them after the CT update, beside this kernel, and the
-// kernel does not wait for the CT.
+// (k_divfree) writes them after the CT update, and this kernel neither waits
+// for the CT nor runs beside it.
 template <int MODE, typename T, int CBS, bool SQ = false, bool MIX = (sizeof(T) == 4), bool LIM = false,
           bool PP = false>
 __global__ __launch_bounds__(CBS, (MIX && LIM && MODE > 0) ? 4 : 1) void k_cell(SRd<MIX> S, const T* __restrict__ FX,
