@@ -133,8 +133,8 @@ __global__ __launch_bounds__(BS) void k_pp_list(SWr<MIX> q, SWr<MIX> out, const 
 // in-cell B of every interior cell from its four new faces, the halo B
 // zeroed.  The limited B planes depend on the faces alone (positivity and
 // TVB changes of Bx, By are overwritten here), so in the mixed form this pass
-// runs after the CT update beside the cell kernel, whose lean epilogue writes
-// the other planes of the limited value
+// runs after the CT update, apart from the cell kernel, whose lean epilogue
+// writes the other planes of the limited value
 template <bool MIX>
 __global__ __launch_bounds__(BS) void k_divfree(SWr<MIX> out, const double* __restrict__ F, Geo g) {
   if (step_off(g)) return;
